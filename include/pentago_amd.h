@@ -1,0 +1,139 @@
+/* pentago_amd: MI355X (gfx950) engine for pentago's retrograde superscore path.
+ *
+ * Plain C ABI: plain pointers and sizes, no C++ or framework types.  Device pointers are
+ * HIP device addresses; `stream` is a hipStream_t passed as void* (NULL = default stream).
+ * Every entry point returns a pe_status; pe_last_error() describes the last failure on the
+ * calling thread.  The reference interface each entry point replaces is cited (paths are
+ * relative to the girving/pentago repository).
+ */
+#ifndef PENTAGO_AMD_H
+#define PENTAGO_AMD_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  PE_OK = 0,
+  PE_ERR_INVALID = 1,     /* bad argument (section, line, board, size) */
+  PE_ERR_DEVICE = 2,      /* HIP runtime error or no device */
+  PE_ERR_TABLES = 3,      /* lookup table self-check failed */
+  PE_ERR_CAPACITY = 4,    /* request exceeds a fixed capacity or budget */
+} pe_status;
+
+/* ---------------- library ---------------- */
+
+/* Build and self-check the lookup tables (SHA-1 against the reference digests of
+ * pentago/base/precompute.cc) and upload them to the current device.
+ * Replaces the generated tables of pentago/base/gen/tables.h. */
+pe_status pe_init(void);
+const char* pe_last_error(void);
+/* SHA-1 digest (40 hex chars + NUL) of a named table, hashed as the reference hashes it. */
+pe_status pe_table_digest(const char* name, char out[41]);
+
+/* ---------------- end engine: block lines ---------------- */
+
+/* section_t: per-quadrant (black, white) stone counts.  pentago/base/section.h:32-33 */
+typedef struct { uint8_t counts[4][2]; } pe_section;
+
+/* line_t: one block line.  pentago/end/line.h:15-35 (dimension:2, length:6, block_base[3]) */
+typedef struct {
+  pe_section section;
+  uint8_t dimension;
+  uint8_t length;          /* ceil(shape[dimension] / 8) */
+  uint8_t block_base[3];   /* block indices of the other three dimensions, in order */
+} pe_line;
+
+/* Rotation-minimal quadrant counts along each dimension: section_t::shape(),
+ * pentago/base/section.cc:43-48. */
+pe_status pe_section_shape(const pe_section* section, int32_t shape[4]);
+/* Standard form under the 8 global symmetries and the symmetry used:
+ * section_t::standardize<8>(), pentago/base/section.cc:89-103. */
+pe_status pe_section_standardize(const pe_section* section, pe_section* standard, int32_t* symmetry);
+
+/* Shapes and sizes derived from a line: pentago/end/compute.cc:24-123
+ * (line_data_t and line_details_t). */
+typedef struct {
+  int32_t input_shape[4];       /* unstandardized child block line; zero at slice 35 */
+  int32_t output_shape[4];
+  int32_t block_stride;         /* nodes per output block (last block may be shorter) */
+  int32_t input_blocks;
+  int32_t length;
+  int32_t dimension;
+  int32_t child_dimension;
+  int64_t input_nodes;          /* input buffer nodes: input_shape product + input_blocks */
+  int64_t output_nodes;
+  pe_section standard_child;
+  int32_t section_transform;
+  int32_t permutation[4];
+  int32_t first_child_block[4];
+} pe_line_info;
+
+/* Nodes are two 256-bit supers (we-win, we-win-or-tie), 64 bytes, little-endian words.
+ * Input layout: input_blocks child blocks, block k at node offset k*(block_stride+1),
+ * C-order within each block (pentago/end/compute.cc:134-139).  Output layout: length
+ * blocks at node offset k*block_stride (compute.cc:147-152). */
+pe_status pe_line_describe(const pe_line* line, pe_line_info* info);
+
+/* Compute one block line on the device (asynchronous on `stream`).
+ * Replaces schedule_compute_line(line_details_t&), pentago/end/compute.h:123. */
+pe_status pe_end_compute_line(const pe_line* line, const void* d_input, void* d_output, void* stream);
+
+/* Same, then call wakeup(ctx) on a runtime thread once the output is complete: the
+ * reference's completion callback line_details_t::wakeup (pentago/end/compute.h:84-86,
+ * compute.cc:359-370). */
+pe_status pe_end_schedule_line(const pe_line* line, const void* d_input, void* d_output, void* stream,
+                               void (*wakeup)(void* ctx), void* ctx);
+
+/* Many lines in one launch (lines of different sections and lengths may be mixed).
+ * The batched form of pentago/end/compute.cc:394-404. */
+pe_status pe_end_compute_lines(int n, const pe_line* lines, const void* const* d_inputs,
+                               void* const* d_outputs, void* stream);
+
+/* Synchronous host-buffer convenience: copies the input to the device, computes, copies the
+ * output back.  Same layouts as above. */
+pe_status pe_end_compute_line_host(const pe_line* line, const void* h_input, void* h_output);
+
+/* Seeded synthetic child data on the device: for node i (counted from `first_node`) and
+ * super word w, known = Bernoulli(0.9) bits (16-term binary expansion, p = 58982/65536) and
+ * win = Bernoulli(1/2) & known, stored as node = (win, known).  Deterministic in
+ * (seed, node index, word). */
+pe_status pe_fill_seeded(void* d_nodes, int64_t nodes, uint64_t seed, int64_t first_node, void* stream);
+
+/* ---------------- midsolve ---------------- */
+
+/* high_board_s: pentago/high/board_c.h:9-12 */
+typedef struct { uint64_t side[2]; uint32_t ply; } pe_high_board;
+/* halfsupers_t: (win, notlose) halves.  pentago/mid/halfsuper_c.h:16-18 */
+typedef struct { uint64_t win[2]; uint64_t notlose[2]; } pe_halfsupers;
+/* (raw board, value) pair of mid_values_t.  pentago/mid/midengine.h:36 */
+typedef struct { uint64_t raw; int32_t value; int32_t pad; } pe_mid_value;
+
+/* Workspace entries needed for boards with at least min_slice stones.
+ * pentago/mid/midengine.cc:95-97 (entries are 32-byte (win, notlose) pairs here). */
+int64_t pe_midsolve_workspace_size(int min_slice);
+
+/* Solve one board (>= 18 stones) on the device: results[0..spots] as in
+ * midsolve_internal, pentago/mid/midengine.h:40.  Synchronous. */
+pe_status pe_midsolve_internal(const pe_high_board* board, pe_halfsupers results[19]);
+
+/* Solve and traverse: values of the board, its children and (if not middle) grandchildren.
+ * Returns the count through *count (<= 163).  pentago/mid/midengine.cc:171-179 (wasm
+ * midsolve) and ios/shared/c_board.h:30-34 (board_midsolve). */
+pe_status pe_midsolve(const pe_high_board* board, pe_mid_value* values, int* count);
+
+/* Batched: n boards of uneven stone counts in one device pass.  results[i][0..18].
+ * mode 0 = level-synchronous launches, 1 = persistent work-queue kernel. */
+pe_status pe_midsolve_batch(int n, const pe_high_board* boards, pe_halfsupers (*results)[19], int mode,
+                            void* stream);
+
+/* Traverse precomputed results into (raw, value) pairs.  pentago/mid/midengine.cc:175-206 */
+pe_status pe_midsolve_traverse(const pe_high_board* board, const pe_halfsupers* results,
+                               pe_mid_value* values, int* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

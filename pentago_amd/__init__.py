@@ -1,0 +1,240 @@
+"""pentago_amd: MI355X engine for pentago's retrograde superscore path.
+
+Python view of the C ABI in include/pentago_amd.h (libpentago_amd.so, HIP for gfx950).
+Names follow the reference (section_t, line_t, schedule_compute_line, midsolve,
+midsolve_internal, high_board_t).  There is no CPU fallback: every compute call runs the
+HIP kernels, and loading fails loudly when the library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpentago_amd.so")
+
+
+class PentagoError(RuntimeError):
+    pass
+
+
+class Section(C.Structure):
+    """section_t: per-quadrant (black, white) counts (pentago/base/section.h:32-33)."""
+    _fields_ = [("c%d%d" % (q, k), C.c_uint8) for q in range(4) for k in range(2)]
+
+    @classmethod
+    def of(cls, counts):
+        s = cls()
+        for q in range(4):
+            setattr(s, "c%d0" % q, counts[q][0])
+            setattr(s, "c%d1" % q, counts[q][1])
+        return s
+
+    @property
+    def counts(self):
+        return tuple((getattr(self, "c%d0" % q), getattr(self, "c%d1" % q)) for q in range(4))
+
+
+class Line(C.Structure):
+    """line_t: one block line (pentago/end/line.h:15-35)."""
+    _fields_ = [("section", Section), ("dimension", C.c_uint8), ("length", C.c_uint8),
+                ("block_base", C.c_uint8 * 3)]
+
+
+class LineInfo(C.Structure):
+    _fields_ = [("input_shape", C.c_int32 * 4), ("output_shape", C.c_int32 * 4),
+                ("block_stride", C.c_int32), ("input_blocks", C.c_int32), ("length", C.c_int32),
+                ("dimension", C.c_int32), ("child_dimension", C.c_int32),
+                ("input_nodes", C.c_int64), ("output_nodes", C.c_int64),
+                ("standard_child", Section), ("section_transform", C.c_int32),
+                ("permutation", C.c_int32 * 4), ("first_child_block", C.c_int32 * 4)]
+
+
+class HighBoard(C.Structure):
+    """high_board_s (pentago/high/board_c.h:9-12)."""
+    _fields_ = [("side", C.c_uint64 * 2), ("ply", C.c_uint32)]
+
+
+class HalfSupers(C.Structure):
+    """halfsupers_t (pentago/mid/halfsuper_c.h:16-18)."""
+    _fields_ = [("win", C.c_uint64 * 2), ("notlose", C.c_uint64 * 2)]
+
+
+class MidValue(C.Structure):
+    _fields_ = [("raw", C.c_uint64), ("value", C.c_int32), ("pad", C.c_int32)]
+
+
+WAKEUP = C.CFUNCTYPE(None, C.c_void_p)
+
+_lib = None
+
+
+def lib():
+    """Load libpentago_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PentagoError("libpentago_amd.so is not built: run __graft_entry__.build() "
+                               "(or make -C pentago_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        P, V = C.POINTER, C.c_void_p
+        sig = {
+            "pe_init": (C.c_int, []),
+            "pe_last_error": (C.c_char_p, []),
+            "pe_table_digest": (C.c_int, [C.c_char_p, C.c_char_p]),
+            "pe_section_shape": (C.c_int, [P(Section), P(C.c_int32)]),
+            "pe_section_standardize": (C.c_int, [P(Section), P(Section), P(C.c_int32)]),
+            "pe_line_describe": (C.c_int, [P(Line), P(LineInfo)]),
+            "pe_end_compute_line": (C.c_int, [P(Line), V, V, V]),
+            "pe_end_schedule_line": (C.c_int, [P(Line), V, V, V, WAKEUP, V]),
+            "pe_end_compute_lines": (C.c_int, [C.c_int, P(Line), P(V), P(V), V]),
+            "pe_end_compute_line_host": (C.c_int, [P(Line), V, V]),
+            "pe_fill_seeded": (C.c_int, [V, C.c_int64, C.c_uint64, C.c_int64, V]),
+            "pe_midsolve_workspace_size": (C.c_int64, [C.c_int]),
+            "pe_midsolve_internal": (C.c_int, [P(HighBoard), P(HalfSupers)]),
+            "pe_midsolve": (C.c_int, [P(HighBoard), P(MidValue), P(C.c_int)]),
+            "pe_midsolve_batch": (C.c_int, [C.c_int, P(HighBoard), V, C.c_int, V]),
+            "pe_midsolve_traverse": (C.c_int, [P(HighBoard), P(HalfSupers), P(MidValue), P(C.c_int)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(status, what):
+    if status != 0:
+        raise PentagoError("%s: %s (status %d)" % (what, lib().pe_last_error().decode(), status))
+
+
+def init():
+    """Build and self-check the tables and upload them to the current device."""
+    _check(lib().pe_init(), "pe_init")
+
+
+def table_digest(name):
+    out = C.create_string_buffer(41)
+    _check(lib().pe_table_digest(name.encode(), out), "pe_table_digest")
+    return out.value.decode()
+
+
+# ---------------- end engine ----------------
+
+def section_shape(counts):
+    """section_t::shape(): rotation-minimal quadrant counts per dimension."""
+    shape = (C.c_int32 * 4)()
+    _check(lib().pe_section_shape(C.byref(Section.of(counts)), shape), "pe_section_shape")
+    return tuple(shape)
+
+
+def standardize(counts):
+    """section_t::standardize<8>(): (standard counts, symmetry)."""
+    out = Section()
+    g = C.c_int32()
+    _check(lib().pe_section_standardize(C.byref(Section.of(counts)), C.byref(out), C.byref(g)),
+           "pe_section_standardize")
+    return out.counts, g.value
+
+
+def make_line(counts, dimension, block_base):
+    """line_t for section `counts`, a dimension and the other three block indices."""
+    line = Line()
+    line.section = Section.of(counts)
+    line.dimension = dimension
+    line.length = (section_shape(counts)[dimension] + 7) // 8
+    for i in range(3):
+        line.block_base[i] = block_base[i]
+    return line
+
+
+def line_describe(line):
+    info = LineInfo()
+    _check(lib().pe_line_describe(C.byref(line), C.byref(info)), "pe_line_describe")
+    return info
+
+
+def compute_line_host(line, inp):
+    """Synchronous: input (uint64, 8 words per node, reference layout) -> output array."""
+    info = line_describe(line)
+    out = np.zeros(8 * info.output_nodes, dtype=np.uint64)
+    ip = inp.ctypes.data if inp is not None and inp.size else None
+    _check(lib().pe_end_compute_line_host(C.byref(line), ip, out.ctypes.data), "pe_end_compute_line_host")
+    return out
+
+
+def compute_lines(lines, d_inputs, d_outputs, stream=None):
+    """Asynchronous batch launch on device pointers."""
+    n = len(lines)
+    arr = (Line * n)(*lines)
+    ins = (C.c_void_p * n)(*d_inputs)
+    outs = (C.c_void_p * n)(*d_outputs)
+    _check(lib().pe_end_compute_lines(n, arr, ins, outs, stream), "pe_end_compute_lines")
+
+
+def schedule_compute_line(line, d_input, d_output, wakeup, stream=None):
+    """schedule_compute_line (pentago/end/compute.h:123): asynchronous, calls wakeup() once
+    the output is complete.  Keep the returned callback object alive until then."""
+    cb = WAKEUP(lambda ctx: wakeup())
+    _check(lib().pe_end_schedule_line(C.byref(line), d_input, d_output, stream, cb, None), "pe_end_schedule_line")
+    return cb
+
+
+def fill_seeded(d_nodes, nodes, seed, first_node=0, stream=None):
+    _check(lib().pe_fill_seeded(d_nodes, nodes, seed, first_node, stream), "pe_fill_seeded")
+
+
+# ---------------- midsolve ----------------
+
+def high_board(side0, side1, ply):
+    b = HighBoard()
+    b.side[0], b.side[1], b.ply = side0, side1, ply
+    return b
+
+
+def high_from_board(board, middle):
+    """high_board_t::from_board (pentago/high/board.cc:208-221)."""
+    s0 = s1 = 0
+    for q in range(4):
+        quad = (board >> (16 * q)) & 0xffff
+        for i in range(9):
+            d = quad % 3
+            quad //= 3
+            if d == 1:
+                s0 |= 1 << (16 * q + i)
+            elif d == 2:
+                s1 |= 1 << (16 * q + i)
+    return high_board(s0, s1, 2 * bin(s0 | s1).count("1") - int(bool(middle)))
+
+
+def _halves(r):
+    return (int(r.win[0]) | int(r.win[1]) << 64, int(r.notlose[0]) | int(r.notlose[1]) << 64)
+
+
+def midsolve_internal(board):
+    """midsolve_internal (pentago/mid/midengine.h:40): list of (win, notlose) halfsupers."""
+    res = (HalfSupers * 19)()
+    _check(lib().pe_midsolve_internal(C.byref(board), res), "pe_midsolve_internal")
+    spots = 36 - ((board.ply + 1) >> 1)
+    return [_halves(res[i]) for i in range(spots + 1)]
+
+
+def midsolve(board):
+    """midsolve (pentago/mid/midengine.h:45): {raw board: value}."""
+    vals = (MidValue * 163)()
+    n = C.c_int()
+    _check(lib().pe_midsolve(C.byref(board), vals, C.byref(n)), "pe_midsolve")
+    return {int(vals[i].raw): int(vals[i].value) for i in range(n.value)}
+
+
+def midsolve_batch(boards, mode=0, stream=None):
+    n = len(boards)
+    arr = (HighBoard * n)(*boards)
+    res = ((HalfSupers * 19) * n)()
+    _check(lib().pe_midsolve_batch(n, arr, C.cast(res, C.c_void_p), mode, stream), "pe_midsolve_batch")
+    out = []
+    for i in range(n):
+        spots = 36 - ((boards[i].ply + 1) >> 1)
+        out.append([_halves(res[i][k]) for k in range(spots + 1)])
+    return out

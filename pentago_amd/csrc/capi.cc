@@ -1,0 +1,284 @@
+// C ABI entry points (include/pentago_amd.h) and device table management.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "device.h"
+#include "end.h"
+#include "mid.h"
+#include "tables.h"
+
+namespace pe {
+
+pe_status launch_lines(const std::vector<LineHost>& lines, hipStream_t stream);
+pe_status launch_fill_seeded(void* d_nodes, int64_t nodes, uint64_t seed, int64_t first, hipStream_t stream);
+size_t mid_board_bytes(const MidInfo& I);
+pe_status mid_solve_levels(const std::vector<MidInfo>& infos, uint8_t* arena, std::vector<size_t>* result_offsets,
+                           hipStream_t stream);
+
+namespace {
+thread_local std::string g_error;
+std::mutex g_dev_mutex;
+constexpr int kMaxDevices = 64;
+DevTables g_dev[kMaxDevices];
+bool g_dev_ready[kMaxDevices];
+}  // namespace
+
+void set_error(const std::string& e) { g_error = e; }
+
+pe_status device_tables(DevTables* out, std::string* err) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) { *err = "no HIP device"; return PE_ERR_DEVICE; }
+  std::lock_guard<std::mutex> lock(g_dev_mutex);
+  if (g_dev_ready[dev]) { *out = g_dev[dev]; return PE_OK; }
+  const Tables* T;
+  try {
+    T = &tables();
+  } catch (const std::exception& e) {
+    *err = e.what();
+    return PE_ERR_TABLES;
+  }
+  // unpacked rotation-minimal quadrants and choose(n, k)
+  std::vector<uint32_t> sides(T->rmin_flat_size);
+  for (int i = 0; i < T->rmin_flat_size; i++)
+    sides[i] = uint32_t(T->unpack[T->rmin_flat[i]][0]) | uint32_t(T->unpack[T->rmin_flat[i]][1]) << 16;
+  std::vector<int32_t> choose(32 * 32);
+  for (int n = 0; n < 32; n++)
+    for (int k = 0; k < 32; k++) choose[n * 32 + k] = choose_host(n, k);
+  const size_t sizes[] = {sizeof T->superwin, sizeof(uint16_t) * size_t(T->rmin_flat_size),
+                          sizeof(uint32_t) * sides.size(), sizeof T->rmin_inverse, sizeof T->pack,
+                          sizeof T->commute, sizeof(int32_t) * choose.size()};
+  const void* srcs[] = {T->superwin, T->rmin_flat, sides.data(), T->rmin_inverse, T->pack, T->commute, choose.data()};
+  size_t total = 0, offs[7];
+  for (int i = 0; i < 7; i++) { offs[i] = total; total += (sizes[i] + 255) & ~size_t(255); }
+  void* d = nullptr;
+  if (hipMalloc(&d, total) != hipSuccess) { *err = "hipMalloc(tables) failed"; return PE_ERR_DEVICE; }
+  for (int i = 0; i < 7; i++)
+    if (hipMemcpy(static_cast<uint8_t*>(d) + offs[i], srcs[i], sizes[i], hipMemcpyHostToDevice) != hipSuccess) {
+      *err = "hipMemcpy(tables) failed";
+      return PE_ERR_DEVICE;
+    }
+  auto* b = static_cast<uint8_t*>(d);
+  DevTables D;
+  D.superwin = reinterpret_cast<const uint64_t*>(b + offs[0]);
+  D.rmin_flat = reinterpret_cast<const uint16_t*>(b + offs[1]);
+  D.rmin_sides = reinterpret_cast<const uint32_t*>(b + offs[2]);
+  D.rmin_inverse = reinterpret_cast<const uint16_t*>(b + offs[3]);
+  D.pack = reinterpret_cast<const uint16_t*>(b + offs[4]);
+  D.commute = reinterpret_cast<const uint8_t*>(b + offs[5]);
+  D.choose = reinterpret_cast<const int32_t*>(b + offs[6]);
+  g_dev[dev] = D;
+  g_dev_ready[dev] = true;
+  *out = D;
+  return PE_OK;
+}
+
+namespace {
+
+pe_status fail(pe_status s, const std::string& e) { g_error = e; return s; }
+
+pe_status make_lines(int n, const pe_line* lines, const void* const* ins, void* const* outs,
+                     std::vector<LineHost>* hosts) {
+  hosts->resize(n);
+  for (int k = 0; k < n; k++) {
+    std::string err;
+    const pe_status s = make_line(lines[k], &(*hosts)[k], &err);
+    if (s != PE_OK) return fail(s, "line " + std::to_string(k) + ": " + err);
+    LineHost& h = (*hosts)[k];
+    if (!outs[k]) return fail(PE_ERR_INVALID, "null output for line " + std::to_string(k));
+    if (!h.dev.slice35 && !ins[k]) return fail(PE_ERR_INVALID, "null input for line " + std::to_string(k));
+    h.dev.input = static_cast<const uint64_t*>(ins[k]);
+    h.dev.output = static_cast<uint64_t*>(outs[k]);
+  }
+  return PE_OK;
+}
+
+struct WakeupCtx { void (*fn)(void*); void* ctx; };
+void wakeup_trampoline(void* p) {
+  auto* w = static_cast<WakeupCtx*>(p);
+  w->fn(w->ctx);
+  delete w;
+}
+
+}  // namespace
+}  // namespace pe
+
+using namespace pe;
+
+extern "C" {
+
+pe_status pe_init(void) {
+  try {
+    tables();
+  } catch (const std::exception& e) {
+    return fail(PE_ERR_TABLES, e.what());
+  }
+  DevTables D;
+  std::string err;
+  const pe_status s = device_tables(&D, &err);
+  if (s != PE_OK) return fail(s, err);
+  return PE_OK;
+}
+
+const char* pe_last_error(void) { return g_error.c_str(); }
+
+pe_status pe_table_digest(const char* name, char out[41]) {
+  try {
+    const std::string d = table_digest(name);
+    if (d.empty()) return fail(PE_ERR_INVALID, std::string("unknown table ") + name);
+    std::memcpy(out, d.c_str(), 41);
+    return PE_OK;
+  } catch (const std::exception& e) {
+    return fail(PE_ERR_TABLES, e.what());
+  }
+}
+
+pe_status pe_section_shape(const pe_section* section, int32_t shape[4]) {
+  if (!section || !shape) return fail(PE_ERR_INVALID, "null argument");
+  Section s;
+  std::memcpy(s.c, section->counts, sizeof s.c);
+  int sh[4];
+  if (!s.valid_counts() || !s.shape(sh)) return fail(PE_ERR_INVALID, "invalid section");
+  for (int q = 0; q < 4; q++) shape[q] = sh[q];
+  return PE_OK;
+}
+
+pe_status pe_section_standardize(const pe_section* section, pe_section* standard, int32_t* symmetry) {
+  if (!section || !standard) return fail(PE_ERR_INVALID, "null argument");
+  Section s, t;
+  std::memcpy(s.c, section->counts, sizeof s.c);
+  if (!s.valid_counts()) return fail(PE_ERR_INVALID, "invalid section");
+  const int g = s.standardize(&t);
+  std::memcpy(standard->counts, t.c, sizeof t.c);
+  if (symmetry) *symmetry = g;
+  return PE_OK;
+}
+
+pe_status pe_line_describe(const pe_line* line, pe_line_info* info) {
+  if (!line || !info) return fail(PE_ERR_INVALID, "null argument");
+  LineHost h;
+  std::string err;
+  const pe_status s = make_line(*line, &h, &err);
+  if (s != PE_OK) return fail(s, err);
+  *info = h.info;
+  return PE_OK;
+}
+
+pe_status pe_end_compute_lines(int n, const pe_line* lines, const void* const* d_inputs, void* const* d_outputs,
+                               void* stream) {
+  if (n < 0 || (n && (!lines || !d_inputs || !d_outputs))) return fail(PE_ERR_INVALID, "bad line batch");
+  std::vector<LineHost> hosts;
+  const pe_status s = make_lines(n, lines, d_inputs, d_outputs, &hosts);
+  if (s != PE_OK) return s;
+  return launch_lines(hosts, static_cast<hipStream_t>(stream));
+}
+
+pe_status pe_end_compute_line(const pe_line* line, const void* d_input, void* d_output, void* stream) {
+  return pe_end_compute_lines(1, line, &d_input, &d_output, stream);
+}
+
+pe_status pe_end_schedule_line(const pe_line* line, const void* d_input, void* d_output, void* stream,
+                               void (*wakeup)(void*), void* ctx) {
+  const pe_status s = pe_end_compute_line(line, d_input, d_output, stream);
+  if (s != PE_OK || !wakeup) return s;
+  auto* w = new WakeupCtx{wakeup, ctx};
+  if (hipLaunchHostFunc(static_cast<hipStream_t>(stream), wakeup_trampoline, w) != hipSuccess) {
+    delete w;
+    return fail(PE_ERR_DEVICE, "hipLaunchHostFunc failed");
+  }
+  return PE_OK;
+}
+
+pe_status pe_end_compute_line_host(const pe_line* line, const void* h_input, void* h_output) {
+  if (!line || !h_output) return fail(PE_ERR_INVALID, "null argument");
+  LineHost h;
+  std::string err;
+  pe_status s = make_line(*line, &h, &err);
+  if (s != PE_OK) return fail(s, err);
+  const size_t in_bytes = size_t(h.info.input_nodes) * 64, out_bytes = size_t(h.info.output_nodes) * 64;
+  if (in_bytes && !h_input) return fail(PE_ERR_INVALID, "null input");
+  void *din = nullptr, *dout = nullptr;
+  PE_HIP(hipMalloc(&dout, out_bytes), "hipMalloc(output)");
+  if (in_bytes) {
+    if (hipMalloc(&din, in_bytes) != hipSuccess) { (void)hipFree(dout); return fail(PE_ERR_DEVICE, "hipMalloc(input)"); }
+    PE_HIP(hipMemcpy(din, h_input, in_bytes, hipMemcpyHostToDevice), "hipMemcpy(input)");
+  }
+  PE_HIP(hipMemset(dout, 0, out_bytes), "hipMemset(output)");
+  s = pe_end_compute_line(line, din, dout, nullptr);
+  if (s == PE_OK) {
+    PE_HIP(hipStreamSynchronize(nullptr), "line compute");
+    PE_HIP(hipMemcpy(h_output, dout, out_bytes, hipMemcpyDeviceToHost), "hipMemcpy(output)");
+  }
+  if (din) (void)hipFree(din);
+  (void)hipFree(dout);
+  return s;
+}
+
+pe_status pe_fill_seeded(void* d_nodes, int64_t nodes, uint64_t seed, int64_t first_node, void* stream) {
+  if (nodes < 0 || (nodes && !d_nodes)) return fail(PE_ERR_INVALID, "bad fill");
+  return launch_fill_seeded(d_nodes, nodes, seed, first_node, static_cast<hipStream_t>(stream));
+}
+
+int64_t pe_midsolve_workspace_size(int min_slice) {
+  if (min_slice < 18 || min_slice > 36) return -1;
+  return bottleneck(36 - min_slice);
+}
+
+pe_status pe_midsolve_batch(int n, const pe_high_board* boards, pe_halfsupers (*results)[19], int mode,
+                            void* stream) {
+  if (n < 0 || (n && (!boards || !results))) return fail(PE_ERR_INVALID, "bad batch");
+  if (mode != 0) return fail(PE_ERR_INVALID, "unsupported batch mode");
+  std::vector<MidInfo> infos(n);
+  size_t bytes = 0;
+  for (int b = 0; b < n; b++) {
+    if (!make_mid_info(boards[b], &infos[b])) return fail(PE_ERR_INVALID, "board " + std::to_string(b) + " is not a valid board with >= 18 stones");
+    bytes += mid_board_bytes(infos[b]);
+  }
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  void* arena = nullptr;
+  PE_HIP(hipMallocAsync(&arena, std::max<size_t>(bytes, 256), s), "hipMallocAsync(mid arena)");
+  std::vector<size_t> offs;
+  pe_status st = mid_solve_levels(infos, static_cast<uint8_t*>(arena), &offs, s);
+  if (st == PE_OK) {
+    std::vector<pe_halfsupers> tmp(19);
+    for (int b = 0; b < n; b++) {
+      PE_HIP(hipMemcpyAsync(results[b], static_cast<uint8_t*>(arena) + offs[b], sizeof(pe_halfsupers) * 19,
+                            hipMemcpyDeviceToHost, s), "hipMemcpyAsync(results)");
+    }
+    PE_HIP(hipStreamSynchronize(s), "midsolve batch");
+    for (int b = 0; b < n; b++)
+      for (int i = infos[b].spots + 1; i < 19; i++) std::memset(&results[b][i], 0, sizeof(pe_halfsupers));
+  }
+  PE_HIP(hipFreeAsync(arena, s), "hipFreeAsync(mid arena)");
+  PE_HIP(hipStreamSynchronize(s), "midsolve batch free");
+  return st;
+}
+
+pe_status pe_midsolve_internal(const pe_high_board* board, pe_halfsupers results[19]) {
+  if (!board || !results) return fail(PE_ERR_INVALID, "null argument");
+  pe_halfsupers (*r)[19] = reinterpret_cast<pe_halfsupers (*)[19]>(results);
+  return pe_midsolve_batch(1, board, r, 0, nullptr);
+}
+
+pe_status pe_midsolve_traverse(const pe_high_board* board, const pe_halfsupers* results, pe_mid_value* values,
+                               int* count) {
+  if (!board || !results || !values || !count) return fail(PE_ERR_INVALID, "null argument");
+  MidInfo I;
+  if (!make_mid_info(*board, &I)) return fail(PE_ERR_INVALID, "invalid board");
+  *count = mid_traverse(*board, results, values);
+  return PE_OK;
+}
+
+pe_status pe_midsolve(const pe_high_board* board, pe_mid_value* values, int* count) {
+  if (!board || !values || !count) return fail(PE_ERR_INVALID, "null argument");
+  pe_halfsupers results[19];
+  const pe_status s = pe_midsolve_internal(board, results);
+  if (s != PE_OK) return s;
+  return pe_midsolve_traverse(board, results, values, count);
+}
+
+}  // extern "C"

@@ -1,0 +1,36 @@
+// Device-resident lookup tables and HIP error plumbing shared by the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/pentago_amd.h"
+
+namespace pe {
+
+// Device copies of the lookup tables (uploaded once per device by pe_init).
+struct DevTables {
+  const uint64_t* superwin;     // 4*512*5 supers
+  const uint16_t* rmin_flat;    // packed radix-3 rotation-minimal quadrants
+  const uint32_t* rmin_sides;   // the same quadrants unpacked: side0 | side1 << 16
+  const uint16_t* rmin_inverse; // 19683 entries
+  const uint16_t* pack;         // 512 entries
+  const uint8_t* commute;       // 8*256 entries
+  const int32_t* choose;        // choose(n, k) at [n*32 + k], n,k < 32
+};
+
+// Upload the tables to the current device (idempotent per device).
+pe_status device_tables(DevTables* out, std::string* err);
+
+void set_error(const std::string& e);
+#define PE_HIP(expr, msg)                                                           \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      ::pe::set_error(std::string(msg) + ": " + hipGetErrorString(e_));             \
+      return PE_ERR_DEVICE;                                                         \
+    }                                                                               \
+  } while (0)
+
+}  // namespace pe

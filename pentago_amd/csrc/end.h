@@ -1,0 +1,64 @@
+// End-engine block lines: host-side index state and device descriptors.
+//
+// A block line (pentago/end/line.h:15-35) is one section, one dimension d and a 3-D block
+// base; it computes, for every node of the line, the moves in quadrant d.  Its input is the
+// child block line in the standardized child section, stored the way the reference stores
+// line.input: input_blocks blocks of block_stride+1 nodes, C-order within each block
+// (pentago/end/compute.cc:84-94,134-139).  Its output is `length` blocks of block_stride
+// nodes (compute.cc:147-152).  A node is two supers (we-win, we-win-or-tie), 64 bytes.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/pentago_amd.h"
+
+namespace pe {
+
+struct Section {
+  uint8_t c[4][2];
+  int sum() const { int t = 0; for (int q = 0; q < 4; q++) t += c[q][0] + c[q][1]; return t; }
+  uint64_t sig() const {
+    uint64_t r = 0;
+    for (int q = 0; q < 4; q++) r |= uint64_t(c[q][0]) << (16 * q) | uint64_t(c[q][1]) << (16 * q + 8);
+    return r;
+  }
+  bool valid_counts() const;
+  bool shape(int out[4]) const;
+  Section transform(int g) const;
+  int standardize(Section* out) const;   // returns g
+  Section child(int q) const { Section s = *this; s.c[q][sum() & 1]++; return s; }
+};
+
+// Uniform per-line state consumed by the device kernel.
+struct LineDev {
+  const uint64_t* input;       // child block line (null at slice 35)
+  uint64_t* output;            // output block line
+  int32_t dim, child_dim, turn, slice35;
+  int32_t length, child_length, block_stride, in_dim_size;
+  int32_t out_shape[4];
+  int32_t perm[4];
+  int32_t first_block[4];
+  int32_t first_child_block[4];
+  int32_t reflection_moves[4];
+  int32_t rmin_off[4], rmin_size[4];   // into the device rmin tables
+  int32_t rs_off[5];                   // reflection symmetry offsets, relative to refl_base
+  int32_t refl_base;                   // offset of this line's symmetries in the launch's byte array
+  int32_t cross_shape[3];              // output shape without the line dimension
+  int32_t cross_count;
+  int32_t inv_global, inv_local;       // inverse_transform (compute.cc:69)
+  int32_t child_coord;                 // quadrant that the move's local rotation lands on after conjugation
+  int32_t pad;
+};
+
+struct LineHost {
+  LineDev dev;
+  pe_line_info info;
+  std::vector<uint8_t> refl_syms;      // all_reflection_symmetries (compute.cc:105-122)
+};
+
+// Build the index state of a line (pentago/end/compute.cc:24-123).  Returns a status and
+// fills `err` on failure.
+pe_status make_line(const pe_line& line, LineHost* out, std::string* err);
+
+}  // namespace pe

@@ -1,0 +1,255 @@
+// End-engine block line kernel for gfx950.
+//
+// One thread computes one output node.  A workgroup of 256 threads takes one line, one
+// index i along the line dimension and 256 consecutive cross-section positions, so the
+// quadrant of the line dimension (rmin[d][i]) and everything derived from it -- the moves,
+// the child index j, its local rotation -- are uniform across the wave, while the other
+// three quadrants vary across lanes.  Consecutive lanes read consecutive child nodes
+// (64 bytes each), so each move's gather is a contiguous run per wave.
+//
+// Per node (pentago/end/compute.cc:308-356):
+//   wins0 = super_wins(side0), wins1 = super_wins(side1)
+//   wins  = OR_m super_wins(side0 + move_m) | rmax(OR_m T_m(~child_m[1]))
+//   nl    = rmax(OR_m T_m(~child_m[0]))
+//   out   = ((wins & ~imm) | (wins0 & ~wins1), (nl & ~imm) | wins0),  imm = wins0 | wins1
+// using rmax(A) | rmax(B) = rmax(A | B) and T_m = rearrange_g o shift_{y_m} with the
+// global part g uniform per line, so the rearrangement and rmax run once per node.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "device.h"
+#include "end.h"
+#include "super.h"
+
+namespace pe {
+
+namespace {
+
+__device__ __forceinline__ int local_add(int a, int b) {
+  return ((((a & 0x33) + (b & 0x33)) & 0x33) + (((a & 0xcc) + (b & 0xcc)) & 0xcc)) & 0xff;
+}
+
+__device__ __forceinline__ int dot4(const int a[4], const int b[4]) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+}
+__device__ __forceinline__ void strides4(const int s[4], int st[4]) {
+  st[3] = 1; st[2] = s[3]; st[1] = s[2] * s[3]; st[0] = s[1] * s[2] * s[3];
+}
+
+__device__ __forceinline__ S4 ld(const uint64_t* p) {
+  const ulonglong2* q = reinterpret_cast<const ulonglong2*>(p);
+  const ulonglong2 a = q[0], b = q[1];
+  S4 r; r.w[0] = a.x; r.w[1] = a.y; r.w[2] = b.x; r.w[3] = b.y;
+  return r;
+}
+__device__ __forceinline__ void st(uint64_t* p, S4 v) {
+  ulonglong2* q = reinterpret_cast<ulonglong2*>(p);
+  q[0] = make_ulonglong2(v.w[0], v.w[1]);
+  q[1] = make_ulonglong2(v.w[2], v.w[3]);
+}
+
+constexpr int kThreads = 256;
+
+__global__ __launch_bounds__(kThreads)
+void end_lines_kernel(const LineDev* __restrict__ lines, const uint8_t* __restrict__ refl,
+                      const int64_t* __restrict__ prefix, int nlines, DevTables T) {
+  // Find this workgroup's line: prefix[l] <= blockIdx.x < prefix[l+1]
+  const int64_t task = blockIdx.x;
+  int lo = 0, hi = nlines;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= task) lo = mid; else hi = mid;
+  }
+  const LineDev& L = lines[lo];
+  const int chunks = (L.cross_count + kThreads - 1) / kThreads;
+  const int64_t local = task - prefix[lo];
+  const int i = int(local / chunks);
+  const int p = int(local % chunks) * kThreads + threadIdx.x;
+  if (p >= L.cross_count) return;
+
+  const int dim = L.dim, cd = L.child_dim;
+  // cross-section coordinates (C-order over the three other dimensions)
+  int full_base[4];
+  {
+    int rem = p;
+    const int c2 = rem % L.cross_shape[2]; rem /= L.cross_shape[2];
+    const int c1 = rem % L.cross_shape[1]; rem /= L.cross_shape[1];
+    const int c[3] = {rem, c1, c2};
+    for (int a = 0, k = 0; a < 4; a++) full_base[a] = a == dim ? 0 : c[k++];
+  }
+  const int turn = L.turn;
+  uint64_t base_side0 = 0, base_side1 = 0;
+  for (int a = 0; a < 4; a++) {
+    if (a == dim) continue;
+    const uint32_t sides = T.rmin_sides[L.rmin_off[a] + 8 * L.first_block[a] + full_base[a]];
+    const uint64_t s0 = sides & 0x1ff, s1 = sides >> 16;
+    base_side0 |= (turn ? s1 : s0) << (16 * a);
+    base_side1 |= (turn ? s0 : s1) << (16 * a);
+  }
+
+  // output index (compute.cc:244-251, 267-269)
+  int obs[4];
+  for (int a = 0; a < 4; a++) obs[a] = L.out_shape[a];
+  obs[dim] = 8;
+  const int length = L.length;
+  const int last_blk = i >> 3;
+  if (last_blk == length - 1) obs[dim] = (L.rmin_size[dim] & 7) ? (L.rmin_size[dim] & 7) : 8;
+  int ost[4]; strides4(obs, ost);
+  const int64_t oidx = int64_t(dot4(ost, full_base)) + int64_t(L.block_stride) * last_blk + int64_t(ost[dim]) * (i & 7);
+
+  // node sides
+  const int qi = L.rmin_off[dim] + i;
+  const uint16_t quad = T.rmin_flat[qi];
+  const uint32_t qs = T.rmin_sides[qi];
+  const uint32_t sq0 = turn ? qs >> 16 : qs & 0x1ff, sq1 = turn ? qs & 0x1ff : qs >> 16;
+  const uint64_t side0 = base_side0 | uint64_t(sq0) << (16 * dim);
+  const uint64_t side1 = base_side1 | uint64_t(sq1) << (16 * dim);
+  const S4 wins0 = super_wins(side0, T.superwin);
+  const S4 wins1 = super_wins(side1, T.superwin);
+  S4 wins = s4_zero(), nl = s4_zero();
+  const uint32_t moves = 511 & ~(sq0 | sq1);
+
+  if (L.slice35) {
+    // compute.cc:329-332, with rmax applied once to the OR of the per-move terms
+    S4 a1 = s4_zero(), a0 = s4_zero();
+    for (uint32_t m = moves; m; m &= m - 1) {
+      const uint64_t move = uint64_t(m & (0u - m)) << (16 * dim);
+      const S4 nw = super_wins(side0 | move, T.superwin);
+      wins = s4_or(wins, nw);
+      a1 = s4_or(a1, s4_andnot(nw, wins1));
+      a0 = s4_or(a0, s4_or(nw, s4_not(wins1)));
+    }
+    wins = s4_or(wins, rmax(a1));
+    nl = rmax(a0);
+  } else {
+    // child base indices (compute.cc:253-271)
+    int ibs[4], fcb[4], rcb[4];
+    for (int a = 0; a < 4; a++) { ibs[a] = obs[L.perm[a]]; fcb[a] = full_base[L.perm[a]]; }
+    ibs[cd] = 8;
+    for (int a = 0; a < 4; a++) {
+      const int fcn = 8 * L.first_child_block[a] + fcb[a];
+      rcb[a] = fcb[a] ^ int(a != cd && fcn < L.reflection_moves[a]);
+    }
+    int ist[4]; strides4(ibs, ist);
+    const int input_stride = ist[cd];
+    const int input_base = dot4(ist, rcb);
+    ibs[cd] = (L.in_dim_size & 7) ? (L.in_dim_size & 7) : 8;
+    int ist2[4]; strides4(ibs, ist2);
+    const int last_input_base = dot4(ist2, rcb);
+    const int line_moves = L.reflection_moves[cd];
+    // local part of base_transform (compute.cc:274-281)
+    const uint8_t* syms = refl + L.refl_base;
+    int local_sym = 0;
+    if (L.rs_off[4] > 0)
+      for (int a = 0; a < 4; a++)
+        if (a != cd) local_sym |= syms[L.rs_off[a] + fcb[a]];
+    const int g = L.inv_global;
+    const int base_local = local_add(L.inv_local, local_sym);
+    const uint64_t* in = L.input;
+    S4 acc1 = s4_zero(), acc0 = s4_zero();
+    for (uint32_t m = moves; m; m &= m - 1) {
+      const uint32_t mv = m & (0u - m);
+      wins = s4_or(wins, super_wins(side0 | uint64_t(mv) << (16 * dim), T.superwin));
+      const int new_q = quad + (1 + turn) * T.pack[mv];
+      const int ir = T.rmin_inverse[new_q];
+      const int j = ir >> 2;
+      const int y = local_add(local_add(T.commute[g * 256 + ((ir & 3) << (2 * dim))], base_local),
+                              syms[L.rs_off[cd] + j]);
+      const int64_t cidx = int64_t((j >> 3) == L.child_length - 1 ? last_input_base : input_base) +
+                           int64_t(L.block_stride + 1) * (j >> 3) +
+                           int64_t(input_stride) * ((j ^ int(j < line_moves)) & 7);
+      const uint64_t* c = in + 8 * cidx;
+      acc0 = s4_or(acc0, shift_local(s4_not(ld(c)), y));
+      acc1 = s4_or(acc1, shift_local(s4_not(ld(c + 4)), y));
+    }
+    wins = s4_or(wins, rmax(rearrange(acc1, g)));
+    nl = rmax(rearrange(acc0, g));
+  }
+  const S4 imm = s4_or(wins0, wins1);
+  uint64_t* o = L.output + 8 * oidx;
+  st(o, s4_or(s4_andnot(wins, imm), s4_andnot(wins0, wins1)));
+  st(o + 4, s4_or(s4_andnot(nl, imm), wins0));
+}
+
+// Seeded child data: known ~ Bernoulli(58982/65536) per bit, win ~ Bernoulli(1/2) & known.
+__host__ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+__global__ void fill_seeded_kernel(uint64_t* __restrict__ nodes, int64_t count, uint64_t seed, int64_t first) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;   // (node, word)
+  if (t >= count * 4) return;
+  const int64_t node = t >> 2;
+  const int w = int(t & 3);
+  const uint64_t key = mix64(seed);
+  const uint64_t ctr = (uint64_t(first + node) * 4 + uint64_t(w)) * 32;
+  // p = 0.1110011001100110 in binary, consumed from the last digit to the first
+  const uint32_t digits = 0xE666;   // bit 15 = first binary digit
+  uint64_t known = 0;
+  for (int k = 0; k < 16; k++) {
+    const uint64_t r = mix64(key ^ (ctr + uint64_t(k)));
+    known = (digits >> k & 1) ? (known | r) : (known & r);
+  }
+  const uint64_t win = mix64(key ^ (ctr + 16)) & known;
+  nodes[8 * node + w] = win;
+  nodes[8 * node + 4 + w] = known;
+}
+
+}  // namespace
+
+pe_status launch_lines(const std::vector<LineHost>& lines, hipStream_t stream) {
+  DevTables T;
+  std::string err;
+  if (device_tables(&T, &err) != PE_OK) { set_error(err); return PE_ERR_DEVICE; }
+  const int n = int(lines.size());
+  if (!n) return PE_OK;
+  // pack descriptors, task prefix and reflection symmetries into one upload
+  std::vector<LineDev> devs(n);
+  std::vector<int64_t> prefix(n + 1, 0);
+  std::vector<uint8_t> refl;
+  for (int k = 0; k < n; k++) {
+    devs[k] = lines[k].dev;
+    devs[k].refl_base = int32_t(refl.size());
+    refl.insert(refl.end(), lines[k].refl_syms.begin(), lines[k].refl_syms.end());
+    const int chunks = (devs[k].cross_count + kThreads - 1) / kThreads;
+    prefix[k + 1] = prefix[k] + int64_t(devs[k].rmin_size[devs[k].dim]) * chunks;
+  }
+  refl.push_back(0);
+  const size_t off_prefix = sizeof(LineDev) * n;
+  const size_t off_refl = off_prefix + sizeof(int64_t) * (n + 1);
+  const size_t bytes = off_refl + refl.size();
+  std::vector<uint8_t> blob(bytes);
+  std::memcpy(blob.data(), devs.data(), off_prefix);
+  std::memcpy(blob.data() + off_prefix, prefix.data(), sizeof(int64_t) * (n + 1));
+  std::memcpy(blob.data() + off_refl, refl.data(), refl.size());
+  void* d = nullptr;
+  PE_HIP(hipMallocAsync(&d, bytes, stream), "hipMallocAsync(line descriptors)");
+  PE_HIP(hipMemcpyAsync(d, blob.data(), bytes, hipMemcpyHostToDevice, stream), "hipMemcpyAsync(line descriptors)");
+  const int64_t tasks = prefix[n];
+  if (tasks > 0x7fffffff) { set_error("too many line tasks in one launch"); return PE_ERR_CAPACITY; }
+  auto* base = static_cast<uint8_t*>(d);
+  hipLaunchKernelGGL(end_lines_kernel, dim3(unsigned(tasks)), dim3(kThreads), 0, stream,
+                     reinterpret_cast<const LineDev*>(base), base + off_refl,
+                     reinterpret_cast<const int64_t*>(base + off_prefix), n, T);
+  PE_HIP(hipGetLastError(), "end_lines_kernel launch");
+  PE_HIP(hipFreeAsync(d, stream), "hipFreeAsync(line descriptors)");
+  return PE_OK;
+}
+
+pe_status launch_fill_seeded(void* d_nodes, int64_t nodes, uint64_t seed, int64_t first, hipStream_t stream) {
+  if (nodes <= 0) return PE_OK;
+  const int64_t threads = nodes * 4;
+  const int64_t blocks = (threads + 255) / 256;
+  if (blocks > 0x7fffffff) { set_error("fill too large"); return PE_ERR_CAPACITY; }
+  hipLaunchKernelGGL(fill_seeded_kernel, dim3(unsigned(blocks)), dim3(256), 0, stream,
+                     static_cast<uint64_t*>(d_nodes), nodes, seed, first);
+  PE_HIP(hipGetLastError(), "fill_seeded_kernel launch");
+  return PE_OK;
+}
+
+}  // namespace pe

@@ -1,0 +1,263 @@
+// 256-bit rotation sets ("supers") and 128-bit parity halves, as register values.
+//
+// A super holds one bit per local rotation r = (r0,r1,r2,r3) in Z_4^4 at bit index
+// r0 + 4 r1 + 16 r2 + 64 r3 (pentago/base/superscore.h:29-39).  Here it is four u64 words:
+// word r3, bit r0 + 4 r1 + 16 r2.  Every coordinate operation below is an in-register
+// shift, delta swap or word rename, so one thread owns a whole super and no lane traffic
+// is needed.
+//
+// Coordinate operations, all of the form X'(x) = X(alpha(x)):
+//   shift_q(X, k):  alpha(x) = x + k e_q          (local rotation of quadrant q)
+//   swap_ab(X):     alpha swaps coordinates a, b
+//   negate(X):      alpha(x) = -x                 (quadrant-local reflection)
+// transform_super(s, C) for s = (g, y) is C(commute[g][x] + y)
+// (pentago/base/symmetry.h:84-91); commute[g] is a signed coordinate permutation, which
+// rearrange<g> applies as a fixed sequence of swaps and a negation.
+#pragma once
+#include <cstdint>
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#define PE_HD __host__ __device__ __forceinline__
+#else
+#define PE_HD inline
+#endif
+
+namespace pe {
+
+struct S4 { uint64_t w[4]; };
+struct H2 { uint64_t w[2]; };
+
+PE_HD S4 s4_zero() { S4 r; r.w[0] = r.w[1] = r.w[2] = r.w[3] = 0; return r; }
+PE_HD S4 s4_not(S4 a) { for (int i = 0; i < 4; i++) a.w[i] = ~a.w[i]; return a; }
+PE_HD S4 s4_or(S4 a, S4 b) { for (int i = 0; i < 4; i++) a.w[i] |= b.w[i]; return a; }
+PE_HD S4 s4_and(S4 a, S4 b) { for (int i = 0; i < 4; i++) a.w[i] &= b.w[i]; return a; }
+PE_HD S4 s4_andnot(S4 a, S4 b) { for (int i = 0; i < 4; i++) a.w[i] &= ~b.w[i]; return a; }
+
+// ---------------- shifts along one coordinate ----------------
+PE_HD uint64_t shift0_word(uint64_t x, int k) {   // rotate each nibble right by k
+  if (!k) return x;
+  const uint64_t ones = 0x1111111111111111ull;
+  return ((x >> k) & (ones * (0xfull >> k))) | ((x << (4 - k)) & (ones * ((0xfull << (4 - k)) & 0xf)));
+}
+PE_HD uint64_t shift1_word(uint64_t x, int k) {   // rotate nibbles within each 16-bit chunk
+  if (!k) return x;
+  const uint64_t ones = 0x0001000100010001ull;
+  const int s = 4 * k;
+  return ((x >> s) & (ones * (0xffffull >> s))) | ((x << (16 - s)) & (ones * ((0xffffull << (16 - s)) & 0xffff)));
+}
+PE_HD uint64_t shift2_word(uint64_t x, int k) {   // rotate 16-bit chunks within the word
+  if (!k) return x;
+  const int s = 16 * k;
+  return (x >> s) | (x << (64 - s));
+}
+PE_HD S4 shift0(S4 a, int k) { for (int i = 0; i < 4; i++) a.w[i] = shift0_word(a.w[i], k & 3); return a; }
+PE_HD S4 shift1(S4 a, int k) { for (int i = 0; i < 4; i++) a.w[i] = shift1_word(a.w[i], k & 3); return a; }
+PE_HD S4 shift2(S4 a, int k) { for (int i = 0; i < 4; i++) a.w[i] = shift2_word(a.w[i], k & 3); return a; }
+PE_HD S4 shift3(S4 a, int k) {
+  S4 r;
+  for (int i = 0; i < 4; i++) r.w[i] = a.w[(i + k) & 3];
+  return r;
+}
+// X'(z) = X(z + y) for a packed local rotation y (2 bits per quadrant).
+PE_HD S4 shift_local(S4 a, int y) {
+  a = shift0(a, y & 3);
+  a = shift1(a, y >> 2 & 3);
+  a = shift2(a, y >> 4 & 3);
+  return shift3(a, y >> 6 & 3);
+}
+PE_HD S4 shift_coord(S4 a, int q, int k) {
+  switch (q) {
+    case 0: return shift0(a, k);
+    case 1: return shift1(a, k);
+    case 2: return shift2(a, k);
+    default: return shift3(a, k & 3);
+  }
+}
+
+// ---------------- coordinate swaps and negation ----------------
+PE_HD uint64_t delta_swap(uint64_t x, uint64_t mask, int s) {
+  const uint64_t t = (x ^ (x >> s)) & mask;
+  return x ^ t ^ (t << s);
+}
+PE_HD S4 swap01(S4 a) {
+  for (int i = 0; i < 4; i++) {
+    a.w[i] = delta_swap(a.w[i], 0x0A0A0A0A0A0A0A0Aull, 3);
+    a.w[i] = delta_swap(a.w[i], 0x00CC00CC00CC00CCull, 6);
+  }
+  return a;
+}
+PE_HD S4 swap02(S4 a) {
+  for (int i = 0; i < 4; i++) {
+    a.w[i] = delta_swap(a.w[i], 0x0000AAAA0000AAAAull, 15);
+    a.w[i] = delta_swap(a.w[i], 0x00000000CCCCCCCCull, 30);
+  }
+  return a;
+}
+PE_HD S4 swap12(S4 a) {
+  for (int i = 0; i < 4; i++) {
+    a.w[i] = delta_swap(a.w[i], 0x0000F0F00000F0F0ull, 12);
+    a.w[i] = delta_swap(a.w[i], 0x00000000FF00FF00ull, 24);
+  }
+  return a;
+}
+PE_HD S4 swap23(S4 a) {
+  S4 r;
+  for (int j = 0; j < 4; j++) {
+    uint64_t v = 0;
+    for (int i = 0; i < 4; i++) v |= (a.w[i] >> (16 * j) & 0xffff) << (16 * i);
+    r.w[j] = v;
+  }
+  return r;
+}
+// Exchange bits between a word pair: hi bit p <- lo bit p+s and lo bit p+s <- hi bit p, p in mask.
+PE_HD void pair_swap(uint64_t& lo, uint64_t& hi, uint64_t mask, int s) {
+  const uint64_t t = ((lo >> s) ^ hi) & mask;
+  hi ^= t;
+  lo ^= t << s;
+}
+PE_HD S4 swap03(S4 a) {
+  pair_swap(a.w[0], a.w[1], 0x5555555555555555ull, 1);
+  pair_swap(a.w[2], a.w[3], 0x5555555555555555ull, 1);
+  pair_swap(a.w[0], a.w[2], 0x3333333333333333ull, 2);
+  pair_swap(a.w[1], a.w[3], 0x3333333333333333ull, 2);
+  return a;
+}
+PE_HD S4 swap13(S4 a) {
+  pair_swap(a.w[0], a.w[1], 0x0F0F0F0F0F0F0F0Full, 4);
+  pair_swap(a.w[2], a.w[3], 0x0F0F0F0F0F0F0F0Full, 4);
+  pair_swap(a.w[0], a.w[2], 0x00FF00FF00FF00FFull, 8);
+  pair_swap(a.w[1], a.w[3], 0x00FF00FF00FF00FFull, 8);
+  return a;
+}
+PE_HD S4 negate(S4 a) {
+  for (int i = 0; i < 4; i++) {
+    uint64_t x = a.w[i];
+    x = delta_swap(x, 0x2222222222222222ull, 2);    // r0: 1 <-> 3
+    x = delta_swap(x, 0x00F000F000F000F0ull, 8);    // r1: 1 <-> 3
+    x = delta_swap(x, 0x00000000FFFF0000ull, 32);   // r2: 1 <-> 3
+    a.w[i] = x;
+  }
+  const uint64_t t = a.w[1]; a.w[1] = a.w[3]; a.w[3] = t;   // r3: 1 <-> 3
+  return a;
+}
+
+// X'(x) = X(commute[g][x]).  Sequences found by exhaustive search over the primitives above
+// and checked bit-exactly against the commute table; applied in order.
+PE_HD S4 rearrange(S4 a, int g) {
+  switch (g & 7) {
+    case 0: return a;
+    case 1: return swap13(swap12(swap01(a)));
+    case 2: return swap12(swap03(a));
+    case 3: return swap23(swap02(swap01(a)));
+    case 4: return negate(swap03(a));
+    case 5: return negate(swap13(swap02(a)));
+    case 6: return negate(swap12(a));
+    default: return negate(swap23(swap01(a)));
+  }
+}
+PE_HD S4 transform_super(int g, int y, S4 c) { return rearrange(shift_local(c, y), g); }
+
+// rmax(f)(a) = OR over the 8 single-quadrant rotations b of f(a+b)
+// (pentago/base/superscore.h:245-306).
+PE_HD S4 rmax(S4 a) {
+  S4 r;
+  for (int i = 0; i < 4; i++) {
+    const uint64_t x = a.w[i];
+    r.w[i] = shift0_word(x, 1) | shift0_word(x, 3) | shift1_word(x, 1) | shift1_word(x, 3) |
+             shift2_word(x, 1) | shift2_word(x, 3) | a.w[(i + 1) & 3] | a.w[(i + 3) & 3];
+  }
+  return r;
+}
+
+// ---------------- super_wins (pentago/base/superscore.cc:38-96) ----------------
+// Table: per (quadrant, state) five supers (horizontal, vertical, diagonal lo, hi, assist).
+// OR-reductions over one coordinate axis:
+PE_HD uint64_t red0(uint64_t x) {
+  x |= ((x >> 1) & 0x5555555555555555ull) | ((x << 1) & 0xAAAAAAAAAAAAAAAAull);
+  x |= ((x >> 2) & 0x3333333333333333ull) | ((x << 2) & 0xCCCCCCCCCCCCCCCCull);
+  return x;
+}
+PE_HD uint64_t red1(uint64_t x) {
+  x |= ((x >> 4) & 0x0F0F0F0F0F0F0F0Full) | ((x << 4) & 0xF0F0F0F0F0F0F0F0ull);
+  x |= ((x >> 8) & 0x00FF00FF00FF00FFull) | ((x << 8) & 0xFF00FF00FF00FF00ull);
+  return x;
+}
+PE_HD uint64_t red2(uint64_t x) {
+  x |= (x >> 16) | (x << 48);
+  x |= (x >> 32) | (x << 32);
+  return x;
+}
+PE_HD S4 reduce0(S4 a) { for (int i = 0; i < 4; i++) a.w[i] = red0(a.w[i]); return a; }
+PE_HD S4 reduce1(S4 a) { for (int i = 0; i < 4; i++) a.w[i] = red1(a.w[i]); return a; }
+PE_HD S4 reduce2(S4 a) { for (int i = 0; i < 4; i++) a.w[i] = red2(a.w[i]); return a; }
+PE_HD S4 reduce3(S4 a) {
+  const uint64_t x = a.w[0] | a.w[1] | a.w[2] | a.w[3];
+  a.w[0] = a.w[1] = a.w[2] = a.w[3] = x;
+  return a;
+}
+
+PE_HD S4 load_s4(const uint64_t* p) { S4 r; for (int i = 0; i < 4; i++) r.w[i] = p[i]; return r; }
+
+// table: 4*512*5 supers as u64 words, entry ((q*512+state)*5+field)*4
+PE_HD S4 super_wins(uint64_t side, const uint64_t* table) {
+  const uint64_t* t0 = table + size_t((0 * 512 + (side & 0x1ff)) * 5) * 4;
+  const uint64_t* t1 = table + size_t((1 * 512 + (side >> 16 & 0x1ff)) * 5) * 4;
+  const uint64_t* t2 = table + size_t((2 * 512 + (side >> 32 & 0x1ff)) * 5) * 4;
+  const uint64_t* t3 = table + size_t((3 * 512 + (side >> 48 & 0x1ff)) * 5) * 4;
+  enum { H = 0, V = 4, LO = 8, HI = 12, AS = 16 };
+  S4 wins = s4_zero();
+  wins = s4_or(wins, reduce2(reduce3(s4_and(load_s4(t0 + V), load_s4(t1 + V)))));
+  wins = s4_or(wins, reduce1(reduce3(s4_and(load_s4(t0 + H), load_s4(t2 + H)))));
+  wins = s4_or(wins, reduce2(reduce0(s4_and(load_s4(t1 + H), load_s4(t3 + H)))));
+  wins = s4_or(wins, reduce1(reduce0(s4_and(load_s4(t2 + V), load_s4(t3 + V)))));
+  wins = s4_or(wins, reduce1(s4_and(s4_and(load_s4(t0 + LO), load_s4(t2 + AS)), load_s4(t3 + LO))));
+  wins = s4_or(wins, reduce2(s4_and(s4_and(load_s4(t0 + HI), load_s4(t1 + AS)), load_s4(t3 + HI))));
+  wins = s4_or(wins, reduce3(s4_and(s4_and(load_s4(t1 + LO), load_s4(t0 + AS)), load_s4(t2 + LO))));
+  wins = s4_or(wins, reduce0(s4_and(s4_and(load_s4(t1 + HI), load_s4(t3 + AS)), load_s4(t2 + HI))));
+  return wins;
+}
+
+// ---------------- halfsupers (pentago/mid/halfsuper.h) ----------------
+// h(p)(a,b,c,d) = s(2a+(b+c+d+p)%2, b, c, d) at bit a + 2b + 8c + 32d.
+PE_HD H2 h2_not(H2 a) { a.w[0] = ~a.w[0]; a.w[1] = ~a.w[1]; return a; }
+PE_HD H2 h2_or(H2 a, H2 b) { a.w[0] |= b.w[0]; a.w[1] |= b.w[1]; return a; }
+PE_HD H2 h2_and(H2 a, H2 b) { a.w[0] &= b.w[0]; a.w[1] &= b.w[1]; return a; }
+
+PE_HD H2 split_half(S4 s, int parity) {
+  H2 h; h.w[0] = h.w[1] = 0;
+  const uint64_t odd_nibbles = 0x0F0FF0F00F0FF0F0ull;   // nibbles r1 + 4 r2 with r1 + r2 odd
+  for (int d = 0; d < 4; d++) {
+    const uint64_t x = s.w[d];
+    const uint64_t take_odd = ((d + parity) & 1) ? ~odd_nibbles : odd_nibbles;
+    uint64_t y = (x & 0x5555555555555555ull & ~take_odd) | ((x >> 1) & 0x5555555555555555ull & take_odd);
+    y = (y | (y >> 1)) & 0x3333333333333333ull;
+    y = (y | (y >> 2)) & 0x0F0F0F0F0F0F0F0Full;
+    y = (y | (y >> 4)) & 0x00FF00FF00FF00FFull;
+    y = (y | (y >> 8)) & 0x0000FFFF0000FFFFull;
+    y = (y | (y >> 16)) & 0x00000000FFFFFFFFull;
+    h.w[d >> 1] |= y << (32 * (d & 1));
+  }
+  return h;
+}
+PE_HD H2 halfsuper_wins(uint64_t side, int parity, const uint64_t* table) {
+  return split_half(super_wins(side, table), parity);
+}
+// rmax on halves; flips parity (pentago/mid/halfsuper.h:18-42, 155-203).
+PE_HD uint64_t hrmax_lo(uint64_t x) {
+  const uint64_t t = (x | (x >> 1)) & 0x5555555555555555ull;
+  const uint64_t y0 = t | (t << 1);                               // parity quadrant
+  const uint64_t y1 = ((x << 2) & 0xFCFCFCFCFCFCFCFCull) | ((x << 6) & 0xC0C0C0C0C0C0C0C0ull) |
+                      ((x >> 2) & 0x3F3F3F3F3F3F3F3Full) | ((x >> 6) & 0x0303030303030303ull);
+  const uint64_t y2 = ((x << 8) & 0xFFFFFF00FFFFFF00ull) | ((x << 24) & 0xFF000000FF000000ull) |
+                      ((x >> 8) & 0x00FFFFFF00FFFFFFull) | ((x >> 24) & 0x000000FF000000FFull);
+  return y0 | y1 | y2;
+}
+PE_HD H2 hrmax(H2 h) {
+  const uint64_t a = h.w[0], b = h.w[1];
+  const uint64_t y3 = (a << 32) | (a >> 32) | (b << 32) | (b >> 32);
+  H2 r; r.w[0] = hrmax_lo(a) | y3; r.w[1] = hrmax_lo(b) | y3;
+  return r;
+}
+
+}  // namespace pe

@@ -1,0 +1,90 @@
+"""The C-ABI library loads, exports every entry point the header declares, and its host
+side (tables, sections, line index state) agrees with the oracle.  No GPU calls."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pentago_amd as P
+from oracle import oracle as O
+from tests.common import golden
+from tests.seeded import seeded_nodes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "pentago_amd.h")).read()
+    return sorted(set(re.findall(r"^(?:pe_status|int64_t|const char\*)\s+(pe_[a-z_0-9]+)\s*\(", text, re.M)))
+
+
+def test_exports_every_declared_symbol():
+    lib = C.CDLL(P.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(lib, name), name
+
+
+def test_product_tables_match_reference_digests():
+    for name, digest in golden("table_hashes.json").items():
+        if name == "superstandardize_table":
+            continue   # superstandardize is not on this path
+        assert P.table_digest(name) == digest, name
+
+
+SECTIONS = [((3, 4), (4, 4), (4, 4), (4, 3)), ((4, 4), (4, 3), (3, 4), (4, 4)), ((3, 3), (3, 3), (3, 3), (3, 3)),
+            ((4, 2), (2, 4), (3, 3), (3, 3)), ((5, 4), (4, 5), (4, 4), (5, 4)), ((4, 5), (5, 4), (4, 4), (4, 4)),
+            ((2, 3), (4, 4), (3, 2), (4, 4)), ((4, 4), (4, 4), (4, 4), (5, 4)), ((3, 2), (2, 3), (3, 3), (4, 4))]
+
+
+def test_sections_match_oracle():
+    for c in SECTIONS:
+        assert P.section_shape(c) == O.section_shape(c)
+        assert P.standardize(c) == O.standardize(c)
+
+
+def test_line_index_state_matches_oracle():
+    for c in SECTIONS:
+        shape = P.section_shape(c)
+        for d in range(4):
+            if sum(c[d]) >= 9:
+                continue
+            for base in [(0, 0, 0), tuple((shape[a] - 1) // 8 for a in range(4) if a != d)]:
+                pi = P.line_describe(P.make_line(c, d, base))
+                oi = O.line_describe(O.make_line(c, d, base))
+                for f in ("input_shape", "output_shape", "permutation", "first_child_block"):
+                    assert list(getattr(pi, f)) == list(getattr(oi, f)), (c, d, f)
+                for f in ("block_stride", "input_blocks", "length", "input_nodes", "output_nodes", "section_transform"):
+                    assert getattr(pi, f) == getattr(oi, f), (c, d, f)
+                assert pi.child_dimension == oi.child_dim
+                assert pi.standard_child.counts == O.section_counts(oi.standard_child)
+
+
+def test_invalid_lines_are_rejected():
+    line = P.make_line(((3, 4), (4, 4), (4, 4), (4, 3)), 0, (0, 0, 0))
+    line.length = 7
+    with pytest.raises(P.PentagoError):
+        P.line_describe(line)
+    line = P.make_line(((3, 4), (4, 4), (4, 4), (4, 3)), 0, (30, 0, 0))
+    with pytest.raises(P.PentagoError):
+        P.line_describe(line)
+    with pytest.raises(P.PentagoError):
+        P.section_shape(((9, 1), (0, 0), (0, 0), (0, 0)))
+
+
+def test_midsolve_workspace_sizes():
+    g = golden("bottleneck.json")
+    for sl, size in zip(g["slices"], g["sizes"]):
+        assert P.lib().pe_midsolve_workspace_size(sl) == size
+
+
+def test_seeded_generator_statistics():
+    x = seeded_nodes(4096, 7).reshape(-1, 2, 4)
+    win, known = x[:, 0], x[:, 1]
+    assert not np.any(win & ~known)
+    bits = lambda a: np.unpackbits(a.view(np.uint8)).mean()
+    assert abs(bits(known) - 58982 / 65536) < 0.003
+    assert abs(bits(win) - 58982 / 65536 / 2) < 0.003

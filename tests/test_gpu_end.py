@@ -1,0 +1,198 @@
+"""End-engine block lines on the GPU against the oracle, bit for bit.
+
+Mirrors pentago/end/compute.cc semantics: every output word of a line must equal the
+oracle's for the same input buffer (reference layout).  Real-data pins: production sparse
+samples at slices 34 and 35 (pentago/high/high_test.cc:47-57) computed end to end on the
+GPU.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import pentago_amd as P
+from pentago_amd import hip
+from oracle import oracle as O
+from tests.common import golden
+from tests.seeded import seeded_nodes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    P.init()
+
+
+# (section counts, dimension, block base selector) -- all 8 global transforms, reflections,
+# partial last blocks and the slice-35 path
+CASES = [
+    (((3, 4), (4, 4), (4, 4), (4, 3)), 0, "last"),
+    (((3, 4), (4, 4), (4, 4), (4, 3)), 2, "first"),   # reflected child (transform 6)
+    (((3, 4), (4, 4), (4, 4), (4, 3)), 3, "last"),
+    (((4, 4), (4, 3), (3, 4), (4, 4)), 0, "last"),    # transform 3
+    (((4, 4), (4, 3), (3, 4), (4, 4)), 3, "last"),    # transform 5
+    (((3, 3), (3, 3), (3, 3), (3, 3)), 1, "last"),    # transform 1
+    (((3, 3), (3, 3), (3, 3), (3, 3)), 3, "last"),    # transform 2
+    (((4, 2), (2, 4), (3, 3), (3, 3)), 0, "last"),    # transform 4
+    (((4, 2), (2, 4), (3, 3), (3, 3)), 2, "first"),
+    (((4, 5), (5, 4), (4, 4), (4, 4)), 3, "first"),   # transform 7, slice 34
+    (((2, 3), (4, 4), (3, 2), (4, 4)), 1, "last"),
+    (((3, 2), (2, 3), (3, 3), (4, 4)), 3, "last"),
+    (((5, 4), (4, 5), (4, 4), (5, 4)), 2, "last"),    # slice 35
+    (((4, 4), (4, 4), (4, 4), (5, 4)), 2, "last"),    # slice 33, transform 5
+]
+
+
+def make_case(counts, dim, where):
+    shape = P.section_shape(counts)
+    base = tuple(0 if where == "first" else (shape[a] - 1) // 8 for a in range(4) if a != dim)
+    return P.make_line(counts, dim, base), O.make_line(counts, dim, base)
+
+
+def line_input(info, seed):
+    return seeded_nodes(int(info.input_nodes), seed) if info.input_nodes else None
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_line_parity(case):
+    counts, dim, where = CASES[case]
+    pl, ol = make_case(counts, dim, where)
+    info = P.line_describe(pl)
+    print("case", case, counts, dim, where, "output nodes", info.output_nodes, flush=True)
+    inp = line_input(info, 1000 + case)
+    gpu = P.compute_line_host(pl, inp)
+    ref = np.zeros(8 * info.output_nodes, dtype=np.uint64)
+    O.compute_lines_threaded([ol], [inp], [ref], 16)
+    assert np.array_equal(gpu, ref)
+
+
+def test_batched_lines_match_single_lines():
+    """pe_end_compute_lines on mixed lines equals one launch per line."""
+    picks = [CASES[i] for i in (0, 3, 5, 7, 12)]
+    lines, infos, ins, outs_b, outs_s, keep = [], [], [], [], [], []
+    for k, (counts, dim, where) in enumerate(picks):
+        pl, _ = make_case(counts, dim, where)
+        info = P.line_describe(pl)
+        inp = line_input(info, 77 + k)
+        din = hip.Buffer(max(64, 64 * info.input_nodes))
+        if inp is not None:
+            din.upload(inp)
+        ob, os_ = hip.Buffer(64 * info.output_nodes), hip.Buffer(64 * info.output_nodes)
+        ob.zero()
+        os_.zero()
+        lines.append(pl); infos.append(info); ins.append(din); outs_b.append(ob); outs_s.append(os_)
+    P.compute_lines(lines, [b.ptr for b in ins], [b.ptr for b in outs_b])
+    for pl, din, os_ in zip(lines, ins, outs_s):
+        P.compute_lines([pl], [din.ptr], [os_.ptr])
+    hip.synchronize()
+    for info, ob, os_ in zip(infos, outs_b, outs_s):
+        a = ob.download(np.zeros(8 * info.output_nodes, dtype=np.uint64))
+        b = os_.download(np.zeros(8 * info.output_nodes, dtype=np.uint64))
+        assert np.array_equal(a, b)
+
+
+def test_schedule_line_calls_wakeup_once():
+    counts, dim, where = CASES[12]
+    pl, _ = make_case(counts, dim, where)
+    info = P.line_describe(pl)
+    out = hip.Buffer(64 * info.output_nodes)
+    calls = []
+    cb = P.schedule_compute_line(pl, None, out.ptr, lambda: calls.append(1))
+    hip.synchronize()
+    import time
+    for _ in range(100):
+        if calls:
+            break
+        time.sleep(0.01)
+    assert calls == [1]
+    del cb
+
+
+def test_seeded_fill_matches_host_generator():
+    n = 5000
+    buf = hip.Buffer(64 * n)
+    P.fill_seeded(buf.ptr, n, 12345, 987654321)
+    hip.synchronize()
+    got = buf.download(np.zeros(8 * n, dtype=np.uint64))
+    assert np.array_equal(got, seeded_nodes(n, 12345, 987654321))
+
+
+# ---------------- real data: production samples computed on the GPU ----------------
+
+class GpuBlocks:
+    """Block values of standard sections, computed with GPU lines (one per open dimension,
+    OR-merged as pentago/end/block_store.cc:173-229) from recursively computed child blocks."""
+
+    def __init__(self):
+        self.lines = {}
+
+    def line_output(self, counts, dim, base):
+        key = (counts, dim, base)
+        if key not in self.lines:
+            pl = P.make_line(counts, dim, base)
+            info = P.line_describe(pl)
+            inp = None
+            if info.input_nodes:
+                inp = np.zeros(8 * info.input_nodes, dtype=np.uint64)
+                child = info.standard_child.counts
+                for k in range(info.input_blocks):
+                    cb = list(info.first_child_block)
+                    cb[info.child_dimension] = k
+                    data = self.block(child, tuple(cb))
+                    off = 8 * k * (info.block_stride + 1)
+                    inp[off:off + data.size] = data
+            self.lines[key] = (info, P.compute_line_host(pl, inp))
+        return self.lines[key]
+
+    def block(self, counts, block):
+        shape = P.section_shape(counts)
+        bshape = [min(shape[a], 8 * (block[a] + 1)) - 8 * block[a] for a in range(4)]
+        acc = np.zeros(8 * int(np.prod(bshape)), dtype=np.uint64)
+        for d in range(4):
+            if sum(counts[d]) >= 9:
+                continue
+            base = tuple(block[a] for a in range(4) if a != d)
+            info, out = self.line_output(counts, d, base)
+            off = 8 * info.block_stride * block[d]
+            acc |= out[off:off + acc.size]
+        return acc
+
+
+def board_counts_and_index(board):
+    counts, idx = [], []
+    for q in range(4):
+        quad = (board >> (16 * q)) & 0xffff
+        ir = O.rmin_inverse(quad)
+        assert ir & 3 == 0
+        idx.append(ir >> 2)
+        b = w = 0
+        for i in range(9):
+            d = quad % 3
+            quad //= 3
+            b += d == 1
+            w += d == 2
+        counts.append((b, w))
+    return tuple(counts), idx
+
+
+@pytest.mark.parametrize("slice_", [35, 34])
+def test_production_samples_on_gpu(slice_):
+    data = golden("samples.json")[str(slice_)]
+    G = GpuBlocks()
+    for board, wins in zip(data["boards"], data["wins"]):
+        counts, idx = board_counts_and_index(board)
+        block = tuple(i // 8 for i in idx)
+        shape = P.section_shape(counts)
+        bshape = [min(shape[a], 8 * (block[a] + 1)) - 8 * block[a] for a in range(4)]
+        local = [i % 8 for i in idx]
+        pos = ((local[0] * bshape[1] + local[1]) * bshape[2] + local[2]) * bshape[3] + local[3]
+        node = G.block(counts, block)[8 * pos:8 * pos + 8]
+        win = sum(int(node[i]) << (64 * i) for i in range(4))
+        wot = sum(int(node[4 + i]) << (64 * i) for i in range(4))
+        black, white = win, (~wot) & (2**256 - 1)
+        if slice_ & 1:
+            black, white = white, black
+        expect_b = sum(wins[i] << (64 * i) for i in range(4))
+        expect_w = sum(wins[4 + i] << (64 * i) for i in range(4))
+        assert (black, white) == (expect_b, expect_w), board
