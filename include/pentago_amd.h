@@ -92,6 +92,32 @@ pe_status pe_end_schedule_line(const pe_line* line, const void* d_input, void* d
 pe_status pe_end_compute_lines(int n, const pe_line* lines, const void* const* d_inputs,
                                void* const* d_outputs, void* stream);
 
+/* Line plans: a device-resident set of line jobs built once and launched any number of
+ * times (the batched form of the reference's per-microline thread-pool scheduling,
+ * pentago/end/compute.cc:378-404).  A job may read its child blocks and write its output
+ * blocks at arbitrary node offsets (for example straight out of section-resident slabs),
+ * restrict the computed indices along the line dimension, and OR its result into the output
+ * (the block-store merge of pentago/end/block_store.cc:173-229, done in place on the device).
+ * Jobs in one plan must not write the same output nodes. */
+typedef struct pe_end_plan pe_end_plan;
+typedef struct {
+  pe_line line;
+  const void* input;                    /* device base of the child data (NULL at slice 35) */
+  void* output;                         /* device base of the output */
+  const int64_t* input_block_offsets;   /* host: node offset of input block k (input_blocks
+                                           entries), or NULL for k*(block_stride+1) */
+  const int64_t* output_block_offsets;  /* host: node offset of output block k (length
+                                           entries), or NULL for k*block_stride */
+  int32_t i_begin, i_end;               /* indices along the line dimension; 0,0 = all */
+  int32_t accumulate;                   /* nonzero: OR into the output */
+  int32_t pad;
+} pe_line_job;
+pe_status pe_end_plan_create(int n, const pe_line_job* jobs, pe_end_plan** plan);
+pe_status pe_end_plan_run(const pe_end_plan* plan, void* stream);
+/* Output nodes one run computes. */
+int64_t pe_end_plan_nodes(const pe_end_plan* plan);
+pe_status pe_end_plan_destroy(pe_end_plan* plan);
+
 /* Synchronous host-buffer convenience: copies the input to the device, computes, copies the
  * output back.  Same layouts as above. */
 pe_status pe_end_compute_line_host(const pe_line* line, const void* h_input, void* h_output);

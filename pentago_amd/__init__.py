@@ -50,6 +50,13 @@ class LineInfo(C.Structure):
                 ("permutation", C.c_int32 * 4), ("first_child_block", C.c_int32 * 4)]
 
 
+class LineJob(C.Structure):
+    """pe_line_job: a line plus where its child blocks and output blocks live."""
+    _fields_ = [("line", Line), ("input", C.c_void_p), ("output", C.c_void_p),
+                ("input_block_offsets", C.POINTER(C.c_int64)), ("output_block_offsets", C.POINTER(C.c_int64)),
+                ("i_begin", C.c_int32), ("i_end", C.c_int32), ("accumulate", C.c_int32), ("pad", C.c_int32)]
+
+
 class HighBoard(C.Structure):
     """high_board_s (pentago/high/board_c.h:9-12)."""
     _fields_ = [("side", C.c_uint64 * 2), ("ply", C.c_uint32)]
@@ -89,6 +96,10 @@ def lib():
             "pe_end_schedule_line": (C.c_int, [P(Line), V, V, V, WAKEUP, V]),
             "pe_end_compute_lines": (C.c_int, [C.c_int, P(Line), P(V), P(V), V]),
             "pe_end_compute_line_host": (C.c_int, [P(Line), V, V]),
+            "pe_end_plan_create": (C.c_int, [C.c_int, P(LineJob), P(V)]),
+            "pe_end_plan_run": (C.c_int, [V, V]),
+            "pe_end_plan_nodes": (C.c_int64, [V]),
+            "pe_end_plan_destroy": (C.c_int, [V]),
             "pe_fill_seeded": (C.c_int, [V, C.c_int64, C.c_uint64, C.c_int64, V]),
             "pe_midsolve_workspace_size": (C.c_int64, [C.c_int]),
             "pe_midsolve_internal": (C.c_int, [P(HighBoard), P(HalfSupers)]),
@@ -179,6 +190,48 @@ def schedule_compute_line(line, d_input, d_output, wakeup, stream=None):
     cb = WAKEUP(lambda ctx: wakeup())
     _check(lib().pe_end_schedule_line(C.byref(line), d_input, d_output, stream, cb, None), "pe_end_schedule_line")
     return cb
+
+
+class Plan:
+    """A device-resident set of line jobs (pe_end_plan), launched with run()."""
+
+    def __init__(self, jobs):
+        """jobs: list of dicts with keys line, input, output and optionally in_offsets,
+        out_offsets (sequences of node offsets), i_range (begin, end), accumulate."""
+        n = len(jobs)
+        arr = (LineJob * n)()
+        keep = []
+        for k, j in enumerate(jobs):
+            J = arr[k]
+            J.line = j["line"]
+            J.input = j.get("input")
+            J.output = j["output"]
+            for key, field in (("in_offsets", "input_block_offsets"), ("out_offsets", "output_block_offsets")):
+                if j.get(key) is not None:
+                    a = np.ascontiguousarray(j[key], dtype=np.int64)
+                    keep.append(a)
+                    setattr(J, field, a.ctypes.data_as(C.POINTER(C.c_int64)))
+            if j.get("i_range"):
+                J.i_begin, J.i_end = j["i_range"]
+            J.accumulate = int(bool(j.get("accumulate", 0)))
+        h = C.c_void_p()
+        _check(lib().pe_end_plan_create(n, arr, C.byref(h)), "pe_end_plan_create")
+        self.handle = h
+        self.nodes = lib().pe_end_plan_nodes(h)
+
+    def run(self, stream=None):
+        _check(lib().pe_end_plan_run(self.handle, stream), "pe_end_plan_run")
+
+    def close(self):
+        if self.handle:
+            lib().pe_end_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def fill_seeded(d_nodes, nodes, seed, first_node=0, stream=None):

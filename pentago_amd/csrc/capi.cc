@@ -14,7 +14,9 @@
 
 namespace pe {
 
-pe_status launch_lines(const std::vector<LineHost>& lines, hipStream_t stream);
+pe_status plan_create(const std::vector<LineHost>& lines, EndPlan* plan);
+pe_status plan_run(const EndPlan& plan, hipStream_t stream);
+void plan_destroy(EndPlan* plan);
 pe_status launch_fill_seeded(void* d_nodes, int64_t nodes, uint64_t seed, int64_t first, hipStream_t stream);
 size_t mid_board_bytes(const MidInfo& I);
 pe_status mid_solve_levels(const std::vector<MidInfo>& infos, uint8_t* arena, std::vector<size_t>* result_offsets,
@@ -80,22 +82,6 @@ pe_status device_tables(DevTables* out, std::string* err) {
 namespace {
 
 pe_status fail(pe_status s, const std::string& e) { g_error = e; return s; }
-
-pe_status make_lines(int n, const pe_line* lines, const void* const* ins, void* const* outs,
-                     std::vector<LineHost>* hosts) {
-  hosts->resize(n);
-  for (int k = 0; k < n; k++) {
-    std::string err;
-    const pe_status s = make_line(lines[k], &(*hosts)[k], &err);
-    if (s != PE_OK) return fail(s, "line " + std::to_string(k) + ": " + err);
-    LineHost& h = (*hosts)[k];
-    if (!outs[k]) return fail(PE_ERR_INVALID, "null output for line " + std::to_string(k));
-    if (!h.dev.slice35 && !ins[k]) return fail(PE_ERR_INVALID, "null input for line " + std::to_string(k));
-    h.dev.input = static_cast<const uint64_t*>(ins[k]);
-    h.dev.output = static_cast<uint64_t*>(outs[k]);
-  }
-  return PE_OK;
-}
 
 struct WakeupCtx { void (*fn)(void*); void* ctx; };
 void wakeup_trampoline(void* p) {
@@ -171,10 +157,70 @@ pe_status pe_line_describe(const pe_line* line, pe_line_info* info) {
 pe_status pe_end_compute_lines(int n, const pe_line* lines, const void* const* d_inputs, void* const* d_outputs,
                                void* stream) {
   if (n < 0 || (n && (!lines || !d_inputs || !d_outputs))) return fail(PE_ERR_INVALID, "bad line batch");
-  std::vector<LineHost> hosts;
-  const pe_status s = make_lines(n, lines, d_inputs, d_outputs, &hosts);
+  std::vector<pe_line_job> jobs(n);
+  for (int k = 0; k < n; k++) {
+    std::memset(&jobs[k], 0, sizeof jobs[k]);
+    jobs[k].line = lines[k];
+    jobs[k].input = d_inputs[k];
+    jobs[k].output = d_outputs[k];
+  }
+  pe_end_plan* plan = nullptr;
+  pe_status s = pe_end_plan_create(n, jobs.data(), &plan);
   if (s != PE_OK) return s;
-  return launch_lines(hosts, static_cast<hipStream_t>(stream));
+  s = pe_end_plan_run(plan, stream);
+  // the plan's device memory is freed once this launch has finished
+  if (s == PE_OK && hipStreamSynchronize(static_cast<hipStream_t>(stream)) != hipSuccess)
+    s = fail(PE_ERR_DEVICE, "line compute failed");
+  pe_end_plan_destroy(plan);
+  return s;
+}
+
+struct pe_end_plan { EndPlan p; };
+
+pe_status pe_end_plan_create(int n, const pe_line_job* jobs, pe_end_plan** out) {
+  if (n < 0 || !out || (n && !jobs)) return fail(PE_ERR_INVALID, "bad plan arguments");
+  std::vector<LineHost> hosts(n);
+  for (int k = 0; k < n; k++) {
+    const pe_line_job& J = jobs[k];
+    std::string err;
+    const pe_status s = make_line(J.line, &hosts[k], &err);
+    if (s != PE_OK) return fail(s, "job " + std::to_string(k) + ": " + err);
+    LineHost& h = hosts[k];
+    if (!J.output) return fail(PE_ERR_INVALID, "null output for job " + std::to_string(k));
+    if (!h.dev.slice35 && !J.input) return fail(PE_ERR_INVALID, "null input for job " + std::to_string(k));
+    h.dev.input = static_cast<const uint64_t*>(J.input);
+    h.dev.output = static_cast<uint64_t*>(J.output);
+    const int size = h.dev.rmin_size[h.dev.dim];
+    if (J.i_begin || J.i_end) {
+      if (J.i_begin < 0 || J.i_end > size || J.i_begin >= J.i_end)
+        return fail(PE_ERR_INVALID, "bad index range for job " + std::to_string(k));
+      h.dev.i_begin = J.i_begin;
+      h.dev.i_end = J.i_end;
+    }
+    h.dev.accumulate = J.accumulate ? 1 : 0;
+    if (J.input_block_offsets && !h.dev.slice35)
+      h.in_off.assign(J.input_block_offsets, J.input_block_offsets + h.info.input_blocks);
+    if (J.output_block_offsets) h.out_off.assign(J.output_block_offsets, J.output_block_offsets + h.info.length);
+  }
+  auto* plan = new pe_end_plan();
+  const pe_status s = plan_create(hosts, &plan->p);
+  if (s != PE_OK) { delete plan; return s; }
+  *out = plan;
+  return PE_OK;
+}
+
+pe_status pe_end_plan_run(const pe_end_plan* plan, void* stream) {
+  if (!plan) return fail(PE_ERR_INVALID, "null plan");
+  return plan_run(plan->p, static_cast<hipStream_t>(stream));
+}
+
+int64_t pe_end_plan_nodes(const pe_end_plan* plan) { return plan ? plan->p.output_nodes : -1; }
+
+pe_status pe_end_plan_destroy(pe_end_plan* plan) {
+  if (!plan) return PE_OK;
+  plan_destroy(&plan->p);
+  delete plan;
+  return PE_OK;
 }
 
 pe_status pe_end_compute_line(const pe_line* line, const void* d_input, void* d_output, void* stream) {

@@ -172,6 +172,11 @@ pe_status make_line(const pe_line& line, LineHost* out, std::string* err) {
     D.child_coord = -1;
     for (int q = 0; q < 4; q++) if (c >> (2 * q) & 3) D.child_coord = q;
   }
+  D.i_begin = 0;
+  D.i_end = D.rmin_size[dim];
+  D.accumulate = 0;
+  D.in_off = nullptr;
+  D.out_off = nullptr;
   return PE_OK;
 }
 

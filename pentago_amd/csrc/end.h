@@ -48,13 +48,29 @@ struct LineDev {
   int32_t cross_count;
   int32_t inv_global, inv_local;       // inverse_transform (compute.cc:69)
   int32_t child_coord;                 // quadrant that the move's local rotation lands on after conjugation
+  int32_t i_begin, i_end;              // range of indices along the line dimension to compute
+  int32_t accumulate;                  // OR into the output instead of storing
   int32_t pad;
+  const int64_t* in_off;               // node offset of input block k (null: k*(block_stride+1))
+  const int64_t* out_off;              // node offset of output block k (null: k*block_stride)
 };
 
 struct LineHost {
   LineDev dev;
   pe_line_info info;
   std::vector<uint8_t> refl_syms;      // all_reflection_symmetries (compute.cc:105-122)
+  std::vector<int64_t> in_off, out_off; // optional block offset tables
+};
+
+// A device-resident set of line jobs, built once and launched any number of times.
+struct EndPlan {
+  void* blob = nullptr;        // descriptors, task prefix, symmetries, offset tables
+  size_t bytes = 0;
+  int nlines = 0;
+  int64_t tasks = 0;
+  int device = -1;
+  int64_t output_nodes = 0;    // nodes written per run (sum of the i ranges' cross sections)
+  size_t off_prefix = 0, off_refl = 0;
 };
 
 // Build the index state of a line (pentago/end/compute.cc:24-123).  Returns a status and

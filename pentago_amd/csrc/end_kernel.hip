@@ -66,7 +66,7 @@ void end_lines_kernel(const LineDev* __restrict__ lines, const uint8_t* __restri
   const LineDev& L = lines[lo];
   const int chunks = (L.cross_count + kThreads - 1) / kThreads;
   const int64_t local = task - prefix[lo];
-  const int i = int(local / chunks);
+  const int i = L.i_begin + int(local / chunks);
   const int p = int(local % chunks) * kThreads + threadIdx.x;
   if (p >= L.cross_count) return;
 
@@ -98,7 +98,8 @@ void end_lines_kernel(const LineDev* __restrict__ lines, const uint8_t* __restri
   const int last_blk = i >> 3;
   if (last_blk == length - 1) obs[dim] = (L.rmin_size[dim] & 7) ? (L.rmin_size[dim] & 7) : 8;
   int ost[4]; strides4(obs, ost);
-  const int64_t oidx = int64_t(dot4(ost, full_base)) + int64_t(L.block_stride) * last_blk + int64_t(ost[dim]) * (i & 7);
+  const int64_t oblock = L.out_off ? L.out_off[last_blk] : int64_t(L.block_stride) * last_blk;
+  const int64_t oidx = oblock + int64_t(dot4(ost, full_base)) + int64_t(ost[dim]) * (i & 7);
 
   // node sides
   const int qi = L.rmin_off[dim] + i;
@@ -158,8 +159,8 @@ void end_lines_kernel(const LineDev* __restrict__ lines, const uint8_t* __restri
       const int j = ir >> 2;
       const int y = local_add(local_add(T.commute[g * 256 + ((ir & 3) << (2 * dim))], base_local),
                               syms[L.rs_off[cd] + j]);
-      const int64_t cidx = int64_t((j >> 3) == L.child_length - 1 ? last_input_base : input_base) +
-                           int64_t(L.block_stride + 1) * (j >> 3) +
+      const int64_t cblock = L.in_off ? L.in_off[j >> 3] : int64_t(L.block_stride + 1) * (j >> 3);
+      const int64_t cidx = cblock + int64_t((j >> 3) == L.child_length - 1 ? last_input_base : input_base) +
                            int64_t(input_stride) * ((j ^ int(j < line_moves)) & 7);
       const uint64_t* c = in + 8 * cidx;
       acc0 = s4_or(acc0, shift_local(s4_not(ld(c)), y));
@@ -170,8 +171,14 @@ void end_lines_kernel(const LineDev* __restrict__ lines, const uint8_t* __restri
   }
   const S4 imm = s4_or(wins0, wins1);
   uint64_t* o = L.output + 8 * oidx;
-  st(o, s4_or(s4_andnot(wins, imm), s4_andnot(wins0, wins1)));
-  st(o + 4, s4_or(s4_andnot(nl, imm), wins0));
+  S4 r0 = s4_or(s4_andnot(wins, imm), s4_andnot(wins0, wins1));
+  S4 r1 = s4_or(s4_andnot(nl, imm), wins0);
+  if (L.accumulate) {   // OR-merge with earlier dimensions (pentago/end/block_store.cc:173-229)
+    r0 = s4_or(r0, ld(o));
+    r1 = s4_or(r1, ld(o + 4));
+  }
+  st(o, r0);
+  st(o + 4, r1);
 }
 
 // Seeded child data: known ~ Bernoulli(58982/65536) per bit, win ~ Bernoulli(1/2) & known.
@@ -202,43 +209,82 @@ __global__ void fill_seeded_kernel(uint64_t* __restrict__ nodes, int64_t count, 
 
 }  // namespace
 
-pe_status launch_lines(const std::vector<LineHost>& lines, hipStream_t stream) {
+// Pack line descriptors, the workgroup prefix, reflection symmetries and block offset tables
+// into one device allocation.  Pointers inside the blob are patched to device addresses.
+pe_status plan_create(const std::vector<LineHost>& lines, EndPlan* plan) {
   DevTables T;
   std::string err;
   if (device_tables(&T, &err) != PE_OK) { set_error(err); return PE_ERR_DEVICE; }
   const int n = int(lines.size());
-  if (!n) return PE_OK;
-  // pack descriptors, task prefix and reflection symmetries into one upload
   std::vector<LineDev> devs(n);
   std::vector<int64_t> prefix(n + 1, 0);
   std::vector<uint8_t> refl;
+  std::vector<int64_t> offs;
+  std::vector<int64_t> in_at(n, -1), out_at(n, -1);
+  int64_t out_nodes = 0;
   for (int k = 0; k < n; k++) {
     devs[k] = lines[k].dev;
     devs[k].refl_base = int32_t(refl.size());
     refl.insert(refl.end(), lines[k].refl_syms.begin(), lines[k].refl_syms.end());
+    if (!lines[k].in_off.empty()) { in_at[k] = int64_t(offs.size()); offs.insert(offs.end(), lines[k].in_off.begin(), lines[k].in_off.end()); }
+    if (!lines[k].out_off.empty()) { out_at[k] = int64_t(offs.size()); offs.insert(offs.end(), lines[k].out_off.begin(), lines[k].out_off.end()); }
     const int chunks = (devs[k].cross_count + kThreads - 1) / kThreads;
-    prefix[k + 1] = prefix[k] + int64_t(devs[k].rmin_size[devs[k].dim]) * chunks;
+    const int span = devs[k].i_end - devs[k].i_begin;
+    prefix[k + 1] = prefix[k] + int64_t(span) * chunks;
+    out_nodes += int64_t(span) * devs[k].cross_count;
   }
+  if (prefix[n] > 0x7fffffff) { set_error("too many line tasks in one plan"); return PE_ERR_CAPACITY; }
   refl.push_back(0);
   const size_t off_prefix = sizeof(LineDev) * n;
-  const size_t off_refl = off_prefix + sizeof(int64_t) * (n + 1);
+  const size_t off_offs = (off_prefix + sizeof(int64_t) * (n + 1) + 255) & ~size_t(255);
+  const size_t off_refl = off_offs + sizeof(int64_t) * offs.size();
   const size_t bytes = off_refl + refl.size();
-  std::vector<uint8_t> blob(bytes);
+  void* d = nullptr;
+  PE_HIP(hipMalloc(&d, bytes), "hipMalloc(line plan)");
+  auto* base = static_cast<uint8_t*>(d);
+  const int64_t* doffs = reinterpret_cast<const int64_t*>(base + off_offs);
+  for (int k = 0; k < n; k++) {
+    devs[k].in_off = in_at[k] >= 0 ? doffs + in_at[k] : nullptr;
+    devs[k].out_off = out_at[k] >= 0 ? doffs + out_at[k] : nullptr;
+  }
+  std::vector<uint8_t> blob(bytes, 0);
   std::memcpy(blob.data(), devs.data(), off_prefix);
   std::memcpy(blob.data() + off_prefix, prefix.data(), sizeof(int64_t) * (n + 1));
+  if (!offs.empty()) std::memcpy(blob.data() + off_offs, offs.data(), sizeof(int64_t) * offs.size());
   std::memcpy(blob.data() + off_refl, refl.data(), refl.size());
-  void* d = nullptr;
-  PE_HIP(hipMallocAsync(&d, bytes, stream), "hipMallocAsync(line descriptors)");
-  PE_HIP(hipMemcpyAsync(d, blob.data(), bytes, hipMemcpyHostToDevice, stream), "hipMemcpyAsync(line descriptors)");
-  const int64_t tasks = prefix[n];
-  if (tasks > 0x7fffffff) { set_error("too many line tasks in one launch"); return PE_ERR_CAPACITY; }
-  auto* base = static_cast<uint8_t*>(d);
-  hipLaunchKernelGGL(end_lines_kernel, dim3(unsigned(tasks)), dim3(kThreads), 0, stream,
-                     reinterpret_cast<const LineDev*>(base), base + off_refl,
-                     reinterpret_cast<const int64_t*>(base + off_prefix), n, T);
-  PE_HIP(hipGetLastError(), "end_lines_kernel launch");
-  PE_HIP(hipFreeAsync(d, stream), "hipFreeAsync(line descriptors)");
+  if (hipMemcpy(d, blob.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    set_error("hipMemcpy(line plan) failed");
+    return PE_ERR_DEVICE;
+  }
+  plan->blob = d;
+  plan->bytes = bytes;
+  plan->nlines = n;
+  plan->tasks = prefix[n];
+  plan->output_nodes = out_nodes;
+  plan->off_prefix = off_prefix;
+  plan->off_refl = off_refl;
+  PE_HIP(hipGetDevice(&plan->device), "hipGetDevice");
   return PE_OK;
+}
+
+pe_status plan_run(const EndPlan& plan, hipStream_t stream) {
+  if (!plan.nlines || !plan.tasks) return PE_OK;
+  DevTables T;
+  std::string err;
+  if (device_tables(&T, &err) != PE_OK) { set_error(err); return PE_ERR_DEVICE; }
+  auto* base = static_cast<uint8_t*>(plan.blob);
+  hipLaunchKernelGGL(end_lines_kernel, dim3(unsigned(plan.tasks)), dim3(kThreads), 0, stream,
+                     reinterpret_cast<const LineDev*>(base), base + plan.off_refl,
+                     reinterpret_cast<const int64_t*>(base + plan.off_prefix), plan.nlines, T);
+  PE_HIP(hipGetLastError(), "end_lines_kernel launch");
+  return PE_OK;
+}
+
+void plan_destroy(EndPlan* plan) {
+  if (plan->blob) (void)hipFree(plan->blob);
+  plan->blob = nullptr;
+  plan->nlines = 0;
 }
 
 pe_status launch_fill_seeded(void* d_nodes, int64_t nodes, uint64_t seed, int64_t first, hipStream_t stream) {
