@@ -32,67 +32,29 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_box(P, hip, k, a0):
-    """Allocate the box output and the child slab, and the four per-dimension plans."""
+def build_box(P, hip, k, a0, mode=0):
+    """Allocate the box output slab and the child slab (seeded) and build the plan."""
     shape = P.section_shape(SECTION)
     assert all(8 * (a0 + k) <= s for s in shape), "box must use full blocks"
     info0 = P.line_describe(P.make_line(SECTION, 0, (a0, a0, a0)))
     child = info0.standard_child.counts
     cshape = P.section_shape(child)
-    cd_len = (cshape[info0.child_dimension] + 7) // 8
-    # child slab: blocks (c_cd in [0, cd_len)) x box range on the three other child axes
-    slab_axes = [range(cd_len) if ax == info0.child_dimension else range(a0, a0 + k) for ax in range(4)]
-    slab_dims = [len(r) for r in slab_axes]
+    cd = info0.child_dimension
+    cd_len = (cshape[cd] + 7) // 8
     n_out_blocks = k ** 4
-    n_child_blocks = int(np.prod(slab_dims))
+    n_child_blocks = cd_len * k ** 3
     out = hip.Buffer(64 * 4096 * n_out_blocks)
     kids = hip.Buffer(64 * 4096 * n_child_blocks)
     P.fill_seeded(kids.ptr, 4096 * n_child_blocks, 0x5EED0000 + sum(c * 10 ** i for i, c in enumerate(sum(child, ()))))
     out.zero()
-
-    def child_offset(cb):
-        idx = 0
-        for ax in range(4):
-            r = slab_axes[ax]
-            assert r.start <= cb[ax] < r.stop, (cb, slab_axes)
-            idx = idx * slab_dims[ax] + (cb[ax] - r.start)
-        return 4096 * idx
-
-    def out_offset(b):
-        idx = 0
-        for ax in range(4):
-            idx = idx * k + (b[ax] - a0)
-        return 4096 * idx
-
-    plans = []
-    child_nodes = []
-    for d in range(4):
-        jobs = []
-        for b1 in range(a0, a0 + k):
-            for b2 in range(a0, a0 + k):
-                for b3 in range(a0, a0 + k):
-                    base = (b1, b2, b3)
-                    line = P.make_line(SECTION, d, base)
-                    info = P.line_describe(line)
-                    assert info.standard_child.counts == child
-                    cd = info.child_dimension
-                    in_off = []
-                    for kk in range(info.input_blocks):
-                        cb = list(info.first_child_block)
-                        cb[cd] = kk
-                        in_off.append(child_offset(cb))
-                    full = [0] * 4
-                    for ax, v in zip([ax for ax in range(4) if ax != d], base):
-                        full[ax] = v
-                    out_off = []
-                    for kk in range(info.length):
-                        full[d] = kk
-                        out_off.append(out_offset(full) if a0 <= kk < a0 + k else 0)
-                    jobs.append(dict(line=line, input=kids.ptr, output=out.ptr, in_offsets=in_off,
-                                     out_offsets=out_off, i_range=(8 * a0, 8 * (a0 + k)), accumulate=d > 0))
-        plans.append(P.Plan(jobs))
-        child_nodes.append(cshape[info0.child_dimension] * (8 * k) ** 3)
-    return out, kids, plans, child_nodes, child, n_child_blocks
+    out_slab = P.Slab.of(SECTION, [a0] * 4, [a0 + k] * 4, out.ptr)
+    lo = [0 if a == cd else a0 for a in range(4)]
+    hi = [cd_len if a == cd else a0 + k for a in range(4)]
+    kid_slab = P.Slab.of(child, lo, hi, kids.ptr)
+    plan = P.Plan.slab(out_slab, [kid_slab], mode)
+    # every dimension's lines read the whole child slab once (the box is a cube)
+    child_nodes = cshape[cd] * (8 * k) ** 3
+    return out, kids, plan, child_nodes, child, n_child_blocks
 
 
 def cpu_baseline(budget_s=15.0):
@@ -134,6 +96,7 @@ def main():
     ap.add_argument("--k", type=int, default=0, help="box edge in blocks (0: largest fitting ~200 GB)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", type=int, default=0, help="0: fused block kernel, 1: four line passes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU; --gpus %d is not supported" % args.gpus,
@@ -158,7 +121,7 @@ def main():
     a0 = 0
     log("box k=%d (%d output blocks), free HBM %.1f GB" % (k, k ** 4, free / 1e9))
     t0 = time.time()
-    out, kids, plans, child_nodes, child, n_child_blocks = build_box(P, hip, k, a0)
+    out, kids, plan, child_nodes, child, n_child_blocks = build_box(P, hip, k, a0, args.mode)
     hip.synchronize()
     log("setup %.1f s: %d lines per pass, child slab %d blocks (%.1f GB), box %.1f GB" % (
         time.time() - t0, k ** 3, n_child_blocks,
@@ -166,43 +129,37 @@ def main():
 
     stream = hip.Stream()
     n_out = (8 * k) ** 4
-    ev = [hip.Event() for _ in range(5)]
+    assert plan.nodes == n_out
     for _ in range(args.warmup):
-        for p in plans:
-            p.run(stream.ptr)
+        plan.run(stream.ptr)
     stream.synchronize()
     hip.synchronize()
-    launch_ms = [0.0] * 4
+    ev0, ev1 = hip.Event(), hip.Event()
+    step_ms = []
     t_start = time.time()
-    ev_all0, ev_all1 = hip.Event(), hip.Event()
-    ev_all0.record(stream)
     for step in range(args.steps):
-        ev[0].record(stream)
-        for d, p in enumerate(plans):
-            p.run(stream.ptr)
-            ev[d + 1].record(stream)
-        ev[4].synchronize()    # per-pass kernel durations of this step, on the launch stream
-        for d in range(4):
-            launch_ms[d] += ev[d].elapsed_ms(ev[d + 1])
-        log("step %d: %.1f ms" % (step, ev[0].elapsed_ms(ev[4])))
-    ev_all1.record(stream)
+        ev0.record(stream)
+        plan.run(stream.ptr)
+        ev1.record(stream)
+        ev1.synchronize()
+        step_ms.append(ev0.elapsed_ms(ev1))
+        log("step %d: %.1f ms" % (step, step_ms[-1]))
     stream.synchronize()
     hip.synchronize()
     wall = time.time() - t_start
-    total_ms = ev_all0.elapsed_ms(ev_all1)
-    ms_per_step = total_ms / args.steps
+    ms_per_step = sum(step_ms) / args.steps
     value = n_out / (ms_per_step / 1e3)
-    avg_launch_ms = sum(launch_ms) / (4 * args.steps)
-    # algorithmic bytes per launch (one dimension pass): box output once + that pass's child nodes once
-    alg_bytes = [64 * (n_out + c) for c in child_nodes]
-    achieved = (sum(alg_bytes) / 4) / (avg_launch_ms / 1e3) / 1e9
+    launches = 1 if args.mode == 0 else 4
+    avg_launch_ms = ms_per_step / launches
+    # algorithmic bytes per step: box output once + every dimension's child nodes once
+    alg_step = 64 * (n_out + 4 * child_nodes)
+    achieved = (alg_step / launches) / (avg_launch_ms / 1e3) / 1e9
+    kernel = "fused_blocks_kernel" if args.mode == 0 else "end_lines_kernel"
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "end_lines_kernel_box_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", "%s_box%d_traffic.json" % (kernel, k))
     if os.path.exists(tfile):
         with open(tfile) as f:
-            t = json.load(f)
-        if t.get("k") == k:
-            traffic = t.get("bytes_per_launch")
+            traffic = json.load(f).get("bytes_per_launch")
     result = {
         "metric": "superpositions/s",
         "value": value,
@@ -218,18 +175,17 @@ def main():
         "data": "synthetic (seeded child blocks: known ~ Bernoulli(0.9), win ~ Bernoulli(0.5) & known)",
         "config": {"workload": "end-engine box of %d^4 blocks of section 24-(3,3)^4, child slab of section 25-%s, "
                                "4 dimension passes" % (k, "".join("%d%d" % c for c in child)),
-                   "box_blocks": k ** 4, "output_nodes": n_out, "child_nodes_per_pass": child_nodes[0],
+                   "box_blocks": k ** 4, "output_nodes": n_out, "child_nodes_per_dimension": child_nodes, "mode": "fused" if args.mode == 0 else "line passes",
                    "lines_per_pass": k ** 3},
-        "hbm_GBps_algorithmic": (sum(alg_bytes)) / (ms_per_step / 1e3) / 1e9,
-        "pass_ms": [x / args.steps for x in launch_ms],
+        "hbm_GBps_algorithmic": alg_step / (ms_per_step / 1e3) / 1e9,
+        "step_ms": step_ms,
         "wall_s": wall,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
-                     "kernel": "end_lines_kernel", "bytes_per_launch": sum(alg_bytes) / 4,
+                     "kernel": kernel, "bytes_per_launch": alg_step / launches,
                      "avg_launch_ms": avg_launch_ms},
     }
-    for p in plans:
-        p.close()
+    plan.close()
     del out, kids
     if not args.no_cpu:
         try:

@@ -118,6 +118,26 @@ pe_status pe_end_plan_run(const pe_end_plan* plan, void* stream);
 int64_t pe_end_plan_nodes(const pe_end_plan* plan);
 pe_status pe_end_plan_destroy(pe_end_plan* plan);
 
+/* Section slabs: the device-resident storage unit.  A dense box of blocks [lo, hi) per axis
+ * of one section, block (b - lo) in C-order over (hi - lo), one 4096-node slot per block,
+ * nodes in C-order over the block's own shape (edge blocks use a prefix of their slot). */
+typedef struct {
+  pe_section section;
+  int32_t lo[4], hi[4];
+  void* data;
+} pe_slab;
+
+/* Plan computing every output block of `output` completely (all open dimensions,
+ * OR-merged), reading child blocks from `children` (slabs of the standard child sections;
+ * every child block any line through the box reads must be covered, else PE_ERR_INVALID).
+ * mode 0: one fused kernel per run (each node's dimensions computed together and written
+ *         once -- the line kernel of pentago/end/compute.cc:219-371 for all four lines
+ *         through a block, merged as pentago/end/block_store.cc:173-229);
+ * mode 1: four line passes, the later ones OR-accumulating (for comparison).
+ * The output section must have at most 34 stones. */
+pe_status pe_end_slab_plan_create(const pe_slab* output, int nchildren, const pe_slab* children, int mode,
+                                  pe_end_plan** plan);
+
 /* Synchronous host-buffer convenience: copies the input to the device, computes, copies the
  * output back.  Same layouts as above. */
 pe_status pe_end_compute_line_host(const pe_line* line, const void* h_input, void* h_output);

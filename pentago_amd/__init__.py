@@ -57,6 +57,27 @@ class LineJob(C.Structure):
                 ("i_begin", C.c_int32), ("i_end", C.c_int32), ("accumulate", C.c_int32), ("pad", C.c_int32)]
 
 
+class Slab(C.Structure):
+    """pe_slab: a dense box of blocks [lo, hi) of one section in device memory."""
+    _fields_ = [("section", Section), ("lo", C.c_int32 * 4), ("hi", C.c_int32 * 4), ("data", C.c_void_p)]
+
+    @classmethod
+    def of(cls, counts, lo, hi, ptr):
+        s = cls()
+        s.section = Section.of(counts)
+        for a in range(4):
+            s.lo[a], s.hi[a] = lo[a], hi[a]
+        s.data = ptr
+        return s
+
+    @property
+    def blocks(self):
+        n = 1
+        for a in range(4):
+            n *= self.hi[a] - self.lo[a]
+        return n
+
+
 class HighBoard(C.Structure):
     """high_board_s (pentago/high/board_c.h:9-12)."""
     _fields_ = [("side", C.c_uint64 * 2), ("ply", C.c_uint32)]
@@ -98,6 +119,7 @@ def lib():
             "pe_end_compute_line_host": (C.c_int, [P(Line), V, V]),
             "pe_end_plan_create": (C.c_int, [C.c_int, P(LineJob), P(V)]),
             "pe_end_plan_run": (C.c_int, [V, V]),
+            "pe_end_slab_plan_create": (C.c_int, [P(Slab), C.c_int, P(Slab), C.c_int, P(V)]),
             "pe_end_plan_nodes": (C.c_int64, [V]),
             "pe_end_plan_destroy": (C.c_int, [V]),
             "pe_fill_seeded": (C.c_int, [V, C.c_int64, C.c_uint64, C.c_int64, V]),
@@ -194,6 +216,19 @@ def schedule_compute_line(line, d_input, d_output, wakeup, stream=None):
 
 class Plan:
     """A device-resident set of line jobs (pe_end_plan), launched with run()."""
+
+    @classmethod
+    def slab(cls, output, children, mode=0):
+        """Plan computing every block of the output slab from child slabs (mode 0 fused,
+        mode 1 four line passes)."""
+        self = cls.__new__(cls)
+        n = len(children)
+        arr = (Slab * max(n, 1))(*children)
+        h = C.c_void_p()
+        _check(lib().pe_end_slab_plan_create(C.byref(output), n, arr, mode, C.byref(h)), "pe_end_slab_plan_create")
+        self.handle = h
+        self.nodes = lib().pe_end_plan_nodes(h)
+        return self
 
     def __init__(self, jobs):
         """jobs: list of dicts with keys line, input, output and optionally in_offsets,

@@ -17,6 +17,11 @@ namespace pe {
 pe_status plan_create(const std::vector<LineHost>& lines, EndPlan* plan);
 pe_status plan_run(const EndPlan& plan, hipStream_t stream);
 void plan_destroy(EndPlan* plan);
+pe_status fused_plan_create(const pe_slab& out, int nch, const pe_slab* ch, FusedPlan* plan, std::string* err);
+pe_status fused_plan_run(const FusedPlan& plan, hipStream_t stream);
+void fused_plan_destroy(FusedPlan* plan);
+pe_status slab_line_passes(const pe_slab& out, int nch, const pe_slab* ch, std::vector<EndPlan>* passes,
+                           int64_t* nodes_out, std::string* err);
 pe_status launch_fill_seeded(void* d_nodes, int64_t nodes, uint64_t seed, int64_t first, hipStream_t stream);
 size_t mid_board_bytes(const MidInfo& I);
 pe_status mid_solve_levels(const std::vector<MidInfo>& infos, uint8_t* arena, std::vector<size_t>* result_offsets,
@@ -175,7 +180,12 @@ pe_status pe_end_compute_lines(int n, const pe_line* lines, const void* const* d
   return s;
 }
 
-struct pe_end_plan { EndPlan p; };
+struct pe_end_plan {
+  std::vector<EndPlan> passes;   // line plans run in order
+  FusedPlan fused;               // or one fused block plan
+  bool is_fused = false;
+  int64_t nodes = 0;
+};
 
 pe_status pe_end_plan_create(int n, const pe_line_job* jobs, pe_end_plan** out) {
   if (n < 0 || !out || (n && !jobs)) return fail(PE_ERR_INVALID, "bad plan arguments");
@@ -203,22 +213,55 @@ pe_status pe_end_plan_create(int n, const pe_line_job* jobs, pe_end_plan** out) 
     if (J.output_block_offsets) h.out_off.assign(J.output_block_offsets, J.output_block_offsets + h.info.length);
   }
   auto* plan = new pe_end_plan();
-  const pe_status s = plan_create(hosts, &plan->p);
+  plan->passes.resize(1);
+  const pe_status s = plan_create(hosts, &plan->passes[0]);
   if (s != PE_OK) { delete plan; return s; }
+  plan->nodes = plan->passes[0].output_nodes;
+  *out = plan;
+  return PE_OK;
+}
+
+pe_status pe_end_slab_plan_create(const pe_slab* output, int nchildren, const pe_slab* children, int mode,
+                                  pe_end_plan** out) {
+  if (!output || !out || nchildren < 0 || (nchildren && !children) || !output->data)
+    return fail(PE_ERR_INVALID, "bad slab plan arguments");
+  if (mode != 0 && mode != 1) return fail(PE_ERR_INVALID, "unknown slab plan mode");
+  auto* plan = new pe_end_plan();
+  std::string err;
+  pe_status s;
+  if (mode == 0) {
+    plan->is_fused = true;
+    s = fused_plan_create(*output, nchildren, children, &plan->fused, &err);
+    plan->nodes = plan->fused.output_nodes;
+  } else {
+    s = slab_line_passes(*output, nchildren, children, &plan->passes, &plan->nodes, &err);
+  }
+  if (s != PE_OK) {
+    for (auto& p : plan->passes) plan_destroy(&p);
+    delete plan;
+    return fail(s, err.empty() ? pe_last_error() : err);
+  }
   *out = plan;
   return PE_OK;
 }
 
 pe_status pe_end_plan_run(const pe_end_plan* plan, void* stream) {
   if (!plan) return fail(PE_ERR_INVALID, "null plan");
-  return plan_run(plan->p, static_cast<hipStream_t>(stream));
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  if (plan->is_fused) return fused_plan_run(plan->fused, st);
+  for (const auto& p : plan->passes) {
+    const pe_status s = plan_run(p, st);
+    if (s != PE_OK) return s;
+  }
+  return PE_OK;
 }
 
-int64_t pe_end_plan_nodes(const pe_end_plan* plan) { return plan ? plan->p.output_nodes : -1; }
+int64_t pe_end_plan_nodes(const pe_end_plan* plan) { return plan ? plan->nodes : -1; }
 
 pe_status pe_end_plan_destroy(pe_end_plan* plan) {
   if (!plan) return PE_OK;
-  plan_destroy(&plan->p);
+  for (auto& p : plan->passes) plan_destroy(&p);
+  fused_plan_destroy(&plan->fused);
   delete plan;
   return PE_OK;
 }

@@ -59,6 +59,31 @@ inline int invert_global(int g) { return g ^ (((~g >> 2) & (g & 1)) << 1); }
 
 }  // namespace
 
+void section_field_tables(const Section& s, std::vector<uint64_t>* out, int32_t field_off[4]) {
+  const auto& T = tables();
+  const int turn = s.sum() & 1;
+  out->clear();
+  for (int q = 0; q < 4; q++) {
+    field_off[q] = int32_t(out->size() / 4);
+    const int b = rmin_bucket(s.c[q][0], s.c[q][1]);
+    for (int i = T.rmin_offsets[b]; i < T.rmin_offsets[b + 1]; i++) {
+      const uint16_t quad = T.rmin_flat[i];
+      const uint16_t mover = T.unpack[quad][turn], other = T.unpack[quad][1 - turn];
+      uint64_t fm[5][4] = {};
+      for (int e = 0; e < 9; e++) {
+        if ((mover | other) >> e & 1) continue;
+        const uint64_t* f = T.superwin + size_t((q * 512 + (mover | 1 << e)) * 5) * 4;
+        for (int k = 0; k < 20; k++) fm[k / 4][k % 4] |= f[k];
+      }
+      const uint64_t* f0 = T.superwin + size_t((q * 512 + mover) * 5) * 4;
+      const uint64_t* f1 = T.superwin + size_t((q * 512 + other) * 5) * 4;
+      out->insert(out->end(), f0, f0 + 20);
+      out->insert(out->end(), f1, f1 + 20);
+      for (int f = 0; f < 5; f++) out->insert(out->end(), fm[f], fm[f] + 4);
+    }
+  }
+}
+
 pe_status make_line(const pe_line& line, LineHost* out, std::string* err) {
   const auto& T = tables();
   std::memset(&out->dev, 0, sizeof out->dev);

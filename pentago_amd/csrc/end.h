@@ -77,4 +77,31 @@ struct EndPlan {
 // fills `err` on failure.
 pe_status make_line(const pe_line& line, LineHost* out, std::string* err);
 
+// ---- fused block plans ----
+// Per-section state of the fused kernel.
+struct SecDev {
+  int32_t turn;
+  int32_t rmin_off[4], rmin_size[4];
+  int32_t field_off[4];     // start (in supers) of quadrant q's field table: 15 supers per rmin index
+};
+// One output block: its section, coordinates, the line through it per dimension, output offset.
+struct BlockDev {
+  int32_t sec;
+  int32_t b[4];
+  int32_t line[4];          // -1 if the dimension is closed
+  int64_t out;              // node offset of the block in the output slab
+};
+
+struct FusedPlan {
+  void* blob = nullptr;
+  int nblocks = 0;
+  size_t off_lines = 0, off_secs = 0, off_fields = 0, off_refl = 0;
+  uint64_t* output = nullptr;
+  int64_t output_nodes = 0;
+};
+
+// Field tables of one section (see SecDev): for quadrant q and rotation-minimal index i,
+// supers [F0 (mover's side), F1 (other side), Fm (OR over the mover's moves)] x 5 fields.
+void section_field_tables(const Section& s, std::vector<uint64_t>* out, int32_t field_off[4]);
+
 }  // namespace pe

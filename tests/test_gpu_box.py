@@ -1,5 +1,6 @@
-"""The benchmark's box workload (child blocks read from a section slab, box-restricted index
-ranges, four OR-merged dimension passes) against the oracle on a small box."""
+"""The benchmark's box workload (output and child slabs, fused block kernel or four OR-merged
+line passes) against the oracle's line kernel on small boxes, plus fused slab plans on
+sections with reflected children and partial edge blocks."""
 import numpy as np
 import pytest
 
@@ -7,54 +8,137 @@ import bench
 import pentago_amd as P
 from pentago_amd import hip
 from oracle import oracle as O
+from tests.seeded import seeded_nodes
 
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("k,a0", [(1, 51), (2, 7)])
-def test_box_matches_oracle(k, a0):
+def oracle_block(counts, blk, child_block):
+    """OR over the open dimensions of the oracle's line output for one block."""
+    shape = O.section_shape(counts)
+    bshape = [min(shape[a], 8 * (blk[a] + 1)) - 8 * blk[a] for a in range(4)]
+    acc = np.zeros(8 * int(np.prod(bshape)), dtype=np.uint64)
+    for d in range(4):
+        if sum(counts[d]) >= 9:
+            continue
+        base = tuple(blk[ax] for ax in range(4) if ax != d)
+        ol = O.make_line(counts, d, base)
+        oi = O.line_describe(ol)
+        inp = np.zeros(8 * oi.input_nodes, dtype=np.uint64)
+        child = O.section_counts(oi.standard_child)
+        for kk in range(oi.input_blocks):
+            cb = list(oi.first_child_block)
+            cb[oi.child_dim] = kk
+            data = child_block(child, tuple(cb))
+            off = 8 * kk * (oi.block_stride + 1)
+            inp[off:off + data.size] = data
+        line_out = np.zeros(8 * oi.output_nodes, dtype=np.uint64)
+        O.compute_lines_threaded([ol], [inp], [line_out], 16)
+        off = 8 * oi.block_stride * blk[d]
+        acc |= line_out[off:off + acc.size]
+    return acc
+
+
+class SlabSet:
+    """Host copies of child slabs: block lookup by (section, block coordinates)."""
+
+    def __init__(self):
+        self.slabs = []
+
+    def add(self, counts, lo, hi, data):
+        self.slabs.append((counts, lo, hi, data))
+
+    def block(self, counts, cb):
+        for c, lo, hi, data in self.slabs:
+            if c != counts or any(not (lo[a] <= cb[a] < hi[a]) for a in range(4)):
+                continue
+            idx = 0
+            for a in range(4):
+                idx = idx * (hi[a] - lo[a]) + (cb[a] - lo[a])
+            shape = O.section_shape(counts)
+            bshape = [min(shape[a], 8 * (cb[a] + 1)) - 8 * cb[a] for a in range(4)]
+            return data[idx, :int(np.prod(bshape))].reshape(-1)
+        raise KeyError((counts, cb))
+
+
+@pytest.mark.parametrize("k,a0,mode", [(1, 51, 0), (2, 7, 0), (2, 7, 1)])
+def test_box_matches_oracle(k, a0, mode):
     P.init()
-    out, kids, plans, child_nodes, child, n_child_blocks = bench.build_box(P, hip, k, a0)
-    for p in plans:
-        p.run()
+    out, kids, plan, child_nodes, child, n_child_blocks = bench.build_box(P, hip, k, a0, mode)
+    plan.run()
     hip.synchronize()
     box = out.download(np.zeros(8 * 4096 * k ** 4, dtype=np.uint64)).reshape(k ** 4, 4096, 8)
     slab = kids.download(np.zeros(8 * 4096 * n_child_blocks, dtype=np.uint64)).reshape(n_child_blocks, 4096, 8)
     info0 = P.line_describe(P.make_line(bench.SECTION, 0, (a0, a0, a0)))
-    cd0 = info0.child_dimension
-    cshape = P.section_shape(child)
-    cd_len = (cshape[cd0] + 7) // 8
-    slab_dims = [cd_len if ax == cd0 else k for ax in range(4)]
-    slab_lo = [0 if ax == cd0 else a0 for ax in range(4)]
-
-    def child_block(cb):
-        idx = 0
-        for ax in range(4):
-            idx = idx * slab_dims[ax] + (cb[ax] - slab_lo[ax])
-        bshape = [min(cshape[ax], 8 * (cb[ax] + 1)) - 8 * cb[ax] for ax in range(4)]
-        return slab[idx, :int(np.prod(bshape))].reshape(-1)
-
-    expect = np.zeros_like(box)
-    rng = np.random.default_rng(k)
-    blocks = [tuple(a0 + int(x) for x in rng.integers(k, size=4)) for _ in range(2)]
-    for blk in blocks:
-        acc = np.zeros(8 * 4096, dtype=np.uint64)
-        for d in range(4):
-            base = tuple(blk[ax] for ax in range(4) if ax != d)
-            ol = O.make_line(bench.SECTION, d, base)
-            oi = O.line_describe(ol)
-            inp = np.zeros(8 * oi.input_nodes, dtype=np.uint64)
-            for kk in range(oi.input_blocks):
-                cb = list(oi.first_child_block)
-                cb[oi.child_dim] = kk
-                data = child_block(cb)
-                off = 8 * kk * (oi.block_stride + 1)
-                inp[off:off + data.size] = data
-            line_out = np.zeros(8 * oi.output_nodes, dtype=np.uint64)
-            O.compute_lines_threaded([ol], [inp], [line_out], 16)
-            off = 8 * oi.block_stride * blk[d]
-            acc |= line_out[off:off + acc.size]
+    cd = info0.child_dimension
+    cd_len = (P.section_shape(child)[cd] + 7) // 8
+    S = SlabSet()
+    S.add(child, [0 if a == cd else a0 for a in range(4)], [cd_len if a == cd else a0 + k for a in range(4)], slab)
+    rng = np.random.default_rng(k + 10 * mode)
+    for _ in range(2):
+        blk = tuple(a0 + int(x) for x in rng.integers(k, size=4))
         idx = 0
         for ax in range(4):
             idx = idx * k + (blk[ax] - a0)
-        assert np.array_equal(box[idx].reshape(-1), acc), blk
+        print("mode", mode, "block", blk, flush=True)
+        assert np.array_equal(box[idx].reshape(-1), oracle_block(bench.SECTION, blk, S.block)), blk
+    plan.close()
+
+
+# Output boxes touching partial edge blocks, with children standardized through
+# reflections (transforms 4..7), several child sections.
+SLAB_CASES = [
+    ((4, 2), (2, 4), (3, 3), (3, 3)),     # all dimensions reflect (transform 4)
+    ((4, 4), (4, 3), (3, 4), (4, 4)),     # transforms 3 and 5
+    ((4, 5), (5, 4), (4, 4), (4, 4)),     # slice 34, transform 7
+]
+
+
+@pytest.mark.parametrize("case", range(len(SLAB_CASES)))
+def test_fused_slab_edges_and_reflections(case):
+    P.init()
+    counts = SLAB_CASES[case]
+    shape = P.section_shape(counts)
+    nb = [(s + 7) // 8 for s in shape]
+    lo = [n - 1 for n in nb]                       # last (partial) block along every axis
+    hi = nb
+    out = hip.Buffer(64 * 4096)
+    out.zero()
+    S = SlabSet()
+    children, keep = [], []
+    seen = set()
+    for d in range(4):
+        if sum(counts[d]) >= 9:
+            continue
+        info = P.line_describe(P.make_line(counts, d, tuple(lo[a] for a in range(4) if a != d)))
+        child = info.standard_child.counts
+        cshape = P.section_shape(child)
+        cd = info.child_dimension
+        clo = list(info.first_child_block)
+        chi = [c + 1 for c in clo]
+        clo[cd], chi[cd] = 0, (cshape[cd] + 7) // 8
+        key = (child, tuple(clo), tuple(chi))
+        if key in seen:
+            continue
+        seen.add(key)
+        nblk = int(np.prod([h - l for l, h in zip(clo, chi)]))
+        host = seeded_nodes(4096 * nblk, 31 + len(keep)).reshape(nblk, 4096, 8)
+        buf = hip.Buffer(host.nbytes)
+        buf.upload(host)
+        keep.append(buf)
+        children.append(P.Slab.of(child, clo, chi, buf.ptr))
+        S.add(child, clo, chi, host)
+    plan = P.Plan.slab(P.Slab.of(counts, lo, hi, out.ptr), children, 0)
+    plan.run()
+    hip.synchronize()
+    got = out.download(np.zeros(8 * 4096, dtype=np.uint64))
+    expect = oracle_block(counts, tuple(lo), S.block)
+    assert np.array_equal(got[:expect.size], expect)
+    plan.close()
+
+
+def test_slab_plan_rejects_missing_children():
+    P.init()
+    out = hip.Buffer(64 * 4096)
+    with pytest.raises(P.PentagoError):
+        P.Plan.slab(P.Slab.of(bench.SECTION, [0] * 4, [1] * 4, out.ptr), [], 0)
