@@ -77,9 +77,12 @@ void section_field_tables(const Section& s, std::vector<uint64_t>* out, int32_t 
       }
       const uint64_t* f0 = T.superwin + size_t((q * 512 + mover) * 5) * 4;
       const uint64_t* f1 = T.superwin + size_t((q * 512 + other) * 5) * 4;
-      out->insert(out->end(), f0, f0 + 20);
-      out->insert(out->end(), f1, f1 + 20);
-      for (int f = 0; f < 5; f++) out->insert(out->end(), fm[f], fm[f] + 4);
+      // word-major: for word k, table t (F0, F1, Fm), field f
+      for (int k = 0; k < 4; k++) {
+        for (int f = 0; f < 5; f++) out->push_back(f0[4 * f + k]);
+        for (int f = 0; f < 5; f++) out->push_back(f1[4 * f + k]);
+        for (int f = 0; f < 5; f++) out->push_back(fm[f][k]);
+      }
     }
   }
 }
@@ -200,6 +203,33 @@ pe_status make_line(const pe_line& line, LineHost* out, std::string* err) {
   D.i_begin = 0;
   D.i_end = D.rmin_size[dim];
   D.accumulate = 0;
+  D.reflected = (st & 4) ? 1 : 0;
+  if (D.child_coord != I.child_dimension) { *err = "internal: moved quadrant does not map to the child dimension"; return PE_ERR_INVALID; }
+  // child block strides per output axis (compute.cc:254-271 with the block shape fixed per line)
+  {
+    const int cd = I.child_dimension;
+    int ibs[4];
+    for (int c = 0; c < 4; c++) ibs[c] = c == cd ? 8 : I.output_shape[I.permutation[c]];
+    auto strides = [](const int s[4], int out[4]) {
+      out[3] = 1; out[2] = s[3]; out[1] = s[2] * s[3]; out[0] = s[1] * s[2] * s[3];
+    };
+    int ist[4], ist2[4];
+    strides(ibs, ist);
+    ibs[cd] = (I.input_shape[dim] & 7) ? (I.input_shape[dim] & 7) : 8;
+    strides(ibs, ist2);
+    D.input_stride = ist[cd];
+    for (int a = 0; a < 4; a++) {
+      D.ax_stride[a] = D.ax_stride_last[a] = D.ax_fcn[a] = D.ax_moves[a] = D.ax_rs[a] = 0;
+      if (a == dim) continue;
+      int c = 0;
+      while (I.permutation[c] != a) c++;
+      D.ax_stride[a] = ist[c];
+      D.ax_stride_last[a] = ist2[c];
+      D.ax_fcn[a] = 8 * I.first_child_block[c];
+      D.ax_moves[a] = child_moved[c];
+      D.ax_rs[a] = D.rs_off[c];
+    }
+  }
   D.in_off = nullptr;
   D.out_off = nullptr;
   return PE_OK;

@@ -50,7 +50,12 @@ struct LineDev {
   int32_t child_coord;                 // quadrant that the move's local rotation lands on after conjugation
   int32_t i_begin, i_end;              // range of indices along the line dimension to compute
   int32_t accumulate;                  // OR into the output instead of storing
-  int32_t pad;
+  int32_t reflected;                   // child standardized through a reflection
+  // Per output axis a (zero for the line dimension): the child axis c it maps to is
+  // perm^-1(a); stride of c inside a child block (full / last block), 8*first_child_block[c],
+  // reflection_moves[c] and the offset of c's reflection symmetries.
+  int32_t ax_stride[4], ax_stride_last[4], ax_fcn[4], ax_moves[4], ax_rs[4];
+  int32_t input_stride;                // stride of the child dimension inside a child block
   const int64_t* in_off;               // node offset of input block k (null: k*(block_stride+1))
   const int64_t* out_off;              // node offset of output block k (null: k*block_stride)
 };
@@ -101,7 +106,8 @@ struct FusedPlan {
 };
 
 // Field tables of one section (see SecDev): for quadrant q and rotation-minimal index i,
-// supers [F0 (mover's side), F1 (other side), Fm (OR over the mover's moves)] x 5 fields.
+// 60 words: for each super word k, tables [F0 (mover's side), F1 (other side), Fm (OR over
+// the mover's moves)] x 5 fields (word k of each super).
 void section_field_tables(const Section& s, std::vector<uint64_t>* out, int32_t field_off[4]);
 
 }  // namespace pe

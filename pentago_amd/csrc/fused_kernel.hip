@@ -60,29 +60,33 @@ __device__ __forceinline__ void st(uint64_t* p, S4 v) {
 
 enum { FH = 0, FV = 1, FLO = 2, FHI = 3, FAS = 4 };
 
-// Field f of table t (0: F0, 1: F1, 2: Fm) for quadrant q of this node.
+// Field words of one node: word k of field f of table t (0: F0, 1: F1, 2: Fm), quadrant q.
 struct NodeFields {
   const uint64_t* base[4];
-  __device__ __forceinline__ S4 get(int q, int t, int f) const { return ld(base[q] + 4 * (5 * t + f)); }
+  __device__ __forceinline__ uint64_t w(int q, int k, int t, int f) const { return base[q][15 * k + 5 * t + f]; }
 };
 
-// Three group evaluations at once (X0: wins0, X1: wins1, XM: wins of all moves) for a
-// two-quadrant group {a, b}.
-__device__ __forceinline__ void group2(const NodeFields& F, int a, int fa, int b, int fb, S4& x0, S4& x1, S4& xm) {
-  const S4 a0 = F.get(a, 0, fa), b0 = F.get(b, 0, fb);
-  x0 = s4_and(a0, b0);
-  x1 = s4_and(F.get(a, 1, fa), F.get(b, 1, fb));
-  xm = s4_or(s4_and(F.get(a, 2, fa), b0), s4_and(a0, F.get(b, 2, fb)));
+// Word k of a pattern-group term for the three quantities: x[0] mover's side, x[1] other
+// side, x[2] the mover's side with one quadrant's field replaced by its OR over that
+// quadrant's moves (all moves at once).
+__device__ __forceinline__ void g2(const NodeFields& F, int k, int a, int fa, int b, int fb, uint64_t x[3]) {
+  const uint64_t a0 = F.w(a, k, 0, fa), b0 = F.w(b, k, 0, fb);
+  x[0] = a0 & b0;
+  x[1] = F.w(a, k, 1, fa) & F.w(b, k, 1, fb);
+  x[2] = (F.w(a, k, 2, fa) & b0) | (a0 & F.w(b, k, 2, fb));
 }
-__device__ __forceinline__ void group3(const NodeFields& F, int a, int fa, int b, int fb, int c, int fc,
-                                       S4& x0, S4& x1, S4& xm) {
-  const S4 a0 = F.get(a, 0, fa), b0 = F.get(b, 0, fb), c0 = F.get(c, 0, fc);
-  const S4 bc = s4_and(b0, c0);
-  x0 = s4_and(a0, bc);
-  x1 = s4_and(s4_and(F.get(a, 1, fa), F.get(b, 1, fb)), F.get(c, 1, fc));
-  xm = s4_or(s4_and(F.get(a, 2, fa), bc),
-             s4_and(a0, s4_or(s4_and(F.get(b, 2, fb), c0), s4_and(b0, F.get(c, 2, fc)))));
+__device__ __forceinline__ void g3(const NodeFields& F, int k, int a, int fa, int b, int fb, int c, int fc,
+                                   uint64_t x[3]) {
+  const uint64_t a0 = F.w(a, k, 0, fa), b0 = F.w(b, k, 0, fb), c0 = F.w(c, k, 0, fc);
+  const uint64_t bc = b0 & c0;
+  x[0] = a0 & bc;
+  x[1] = F.w(a, k, 1, fa) & F.w(b, k, 1, fb) & F.w(c, k, 1, fc);
+  x[2] = (F.w(a, k, 2, fa) & bc) | (a0 & ((F.w(b, k, 2, fb) & c0) | (b0 & F.w(c, k, 2, fc))));
 }
+
+// Compiler-only memory fence: keeps the field loads of one word from being hoisted above
+// the previous word's work, which would otherwise keep ~200 loaded values live.
+#define PE_FENCE() asm volatile("" ::: "memory")
 
 constexpr int kThreads = 256;
 constexpr int kParts = 4096 / kThreads;
@@ -94,104 +98,102 @@ void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __r
   const BlockDev& B = blocks[blockIdx.x / kParts];
   const SecDev& S = secs[B.sec];
   const int p = (blockIdx.x % kParts) * kThreads + threadIdx.x;
-  int bs[4];
-  for (int a = 0; a < 4; a++) bs[a] = min(8, S.rmin_size[a] - 8 * B.b[a]);
-  if (p >= bs[0] * bs[1] * bs[2] * bs[3]) return;
+  const int bs0 = min(8, S.rmin_size[0] - 8 * B.b[0]), bs1 = min(8, S.rmin_size[1] - 8 * B.b[1]);
+  const int bs2 = min(8, S.rmin_size[2] - 8 * B.b[2]), bs3 = min(8, S.rmin_size[3] - 8 * B.b[3]);
+  if (p >= bs0 * bs1 * bs2 * bs3) return;
   int loc[4];
   {
     int rem = p;
-    loc[3] = rem % bs[3]; rem /= bs[3];
-    loc[2] = rem % bs[2]; rem /= bs[2];
-    loc[1] = rem % bs[1];
-    loc[0] = rem / bs[1];
+    loc[3] = rem % bs3; rem /= bs3;
+    loc[2] = rem % bs2; rem /= bs2;
+    loc[1] = rem % bs1;
+    loc[0] = rem / bs1;
   }
   const int turn = S.turn;
-  int gi[4];
-  NodeFields F;
-  uint32_t qs[4];
-  for (int a = 0; a < 4; a++) {
-    gi[a] = 8 * B.b[a] + loc[a];
-    qs[a] = T.rmin_sides[S.rmin_off[a] + gi[a]];
-    F.base[a] = fields + 4 * (int64_t(S.field_off[a]) + 15 * int64_t(gi[a]));
-  }
 
-  // ---- wins0, wins1 and the wins of every move, grouped by outer reduction ----
-  S4 w0, w1, wm;
-  {
-    S4 x0, x1, xm, y0, y1, ym, z0, z1, zm;
-    group2(F, 0, FH, 2, FH, x0, x1, xm);                // red3 then red1
-    group2(F, 2, FV, 3, FV, y0, y1, ym);                // red0 then red1
-    group3(F, 0, FLO, 2, FAS, 3, FLO, z0, z1, zm);      // red1
-    w0 = reduce1(s4_or(s4_or(reduce3(x0), reduce0(y0)), z0));
-    w1 = reduce1(s4_or(s4_or(reduce3(x1), reduce0(y1)), z1));
-    wm = reduce1(s4_or(s4_or(reduce3(xm), reduce0(ym)), zm));
-    group2(F, 0, FV, 1, FV, x0, x1, xm);                // red3 then red2
-    group2(F, 1, FH, 3, FH, y0, y1, ym);                // red0 then red2
-    group3(F, 0, FHI, 1, FAS, 3, FHI, z0, z1, zm);      // red2
-    w0 = s4_or(w0, reduce2(s4_or(s4_or(reduce3(x0), reduce0(y0)), z0)));
-    w1 = s4_or(w1, reduce2(s4_or(s4_or(reduce3(x1), reduce0(y1)), z1)));
-    wm = s4_or(wm, reduce2(s4_or(s4_or(reduce3(xm), reduce0(ym)), zm)));
-    group3(F, 1, FLO, 0, FAS, 2, FLO, x0, x1, xm);      // red3
-    group3(F, 1, FHI, 3, FAS, 2, FHI, y0, y1, ym);      // red0
-    w0 = s4_or(w0, s4_or(reduce3(x0), reduce0(y0)));
-    w1 = s4_or(w1, s4_or(reduce3(x1), reduce0(y1)));
-    wm = s4_or(wm, s4_or(reduce3(xm), reduce0(ym)));
-  }
-
-  // ---- children, one line per open dimension ----
+  // ---- children: per open dimension, OR the moves' children shifted along the child
+  //      dimension, then apply the rest of the local rotation and the rearrangement ----
   S4 t0 = s4_zero(), t1 = s4_zero();
+#pragma unroll
   for (int d = 0; d < 4; d++) {
     if (B.line[d] < 0) continue;
     const LineDev& L = lines[B.line[d]];
     const int cd = L.child_dim;
-    int full_base[4];
-    for (int a = 0; a < 4; a++) full_base[a] = a == d ? 0 : loc[a];
-    int obs[4];
-    for (int a = 0; a < 4; a++) obs[a] = a == d ? 8 : bs[a];
-    int ibs[4], fcb[4], rcb[4];
-    for (int a = 0; a < 4; a++) { ibs[a] = obs[L.perm[a]]; fcb[a] = full_base[L.perm[a]]; }
-    for (int a = 0; a < 4; a++) {
-      const int fcn = 8 * L.first_child_block[a] + fcb[a];
-      rcb[a] = fcb[a] ^ int(a != cd && fcn < L.reflection_moves[a]);
-    }
-    int ist[4]; strides4(ibs, ist);
-    const int input_stride = ist[cd];
-    const int input_base = dot4(ist, rcb);
-    ibs[cd] = (L.in_dim_size & 7) ? (L.in_dim_size & 7) : 8;
-    int ist2[4]; strides4(ibs, ist2);
-    const int last_input_base = dot4(ist2, rcb);
-    const int line_moves = L.reflection_moves[cd];
+    int base = 0, base_last = 0, local_sym = 0;
     const uint8_t* syms = refl + L.refl_base;
-    int local_sym = 0;
-    if (L.rs_off[4] > 0)
-      for (int a = 0; a < 4; a++)
-        if (a != cd) local_sym |= syms[L.rs_off[a] + fcb[a]];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      if (a == d) continue;
+      const int x = loc[a] ^ int(L.ax_fcn[a] + loc[a] < L.ax_moves[a]);
+      base += L.ax_stride[a] * x;
+      base_last += L.ax_stride_last[a] * x;
+      if (L.reflected) local_sym |= syms[L.ax_rs[a] + loc[a]];
+    }
     const int g = L.inv_global;
-    const int base_local = local_add(L.inv_local, local_sym);
-    const uint32_t sq0 = turn ? qs[d] >> 16 : qs[d] & 0x1ff, sq1 = turn ? qs[d] & 0x1ff : qs[d] >> 16;
-    const int quad = T.rmin_flat[S.rmin_off[d] + gi[d]];
+    const int gi = 8 * B.b[d] + loc[d];
+    const uint32_t qs = T.rmin_sides[S.rmin_off[d] + gi];
+    const uint32_t sq0 = turn ? qs >> 16 : qs & 0x1ff, sq1 = turn ? qs & 0x1ff : qs >> 16;
+    const int quad = T.rmin_flat[S.rmin_off[d] + gi];
+    const int line_moves = L.reflection_moves[cd];
+    const uint8_t* cd_syms = syms + L.rs_off[cd];
+    const uint64_t* in = L.input;
     S4 acc0 = s4_zero(), acc1 = s4_zero();
     for (uint32_t m = 511 & ~(sq0 | sq1); m; m &= m - 1) {
       const uint32_t mv = m & (0u - m);
       const int ir = T.rmin_inverse[quad + (1 + turn) * T.pack[mv]];
       const int j = ir >> 2;
-      const int y = local_add(local_add(T.commute[g * 256 + ((ir & 3) << (2 * d))], base_local),
-                              syms[L.rs_off[cd] + j]);
-      const int64_t cidx = L.in_off[j >> 3] + int64_t((j >> 3) == L.child_length - 1 ? last_input_base : input_base) +
-                           int64_t(input_stride) * ((j ^ int(j < line_moves)) & 7);
-      const uint64_t* c = L.input + 8 * cidx;
-      acc0 = s4_or(acc0, shift_local(s4_not(ld(c)), y));
-      acc1 = s4_or(acc1, shift_local(s4_not(ld(c + 4)), y));
+      // local rotation of this move restricted to the child dimension
+      const int k = ((T.commute[g * 256 + ((ir & 3) << (2 * d))] + cd_syms[j]) >> (2 * cd)) & 3;
+      const int64_t cidx = L.in_off[j >> 3] + int64_t((j >> 3) == L.child_length - 1 ? base_last : base) +
+                           int64_t(L.input_stride) * ((j ^ int(j < line_moves)) & 7);
+      const uint64_t* c = in + 8 * cidx;
+      acc0 = s4_or(acc0, shift_coord_var(s4_not(ld(c)), cd, k));
+      acc1 = s4_or(acc1, shift_coord_var(s4_not(ld(c + 4)), cd, k));
     }
-    t0 = s4_or(t0, rearrange(acc0, g));
-    t1 = s4_or(t1, rearrange(acc1, g));
+    const int y = local_add(L.inv_local, local_sym);
+    t0 = s4_or(t0, rearrange(shift_local(acc0, y), g));
+    t1 = s4_or(t1, rearrange(shift_local(acc1, y), g));
   }
-  const S4 wins = s4_or(wm, rmax(t1));
-  const S4 nl = rmax(t0);
-  const S4 imm = s4_or(w0, w1);
+  const S4 r0 = rmax(t0), r1 = rmax(t1);
+
+  // ---- wins0, wins1 and the wins of every move (pentago/base/superscore.cc:38-96 pattern
+  //      groups), one 64-bit word at a time: every reduction except the one over r3 stays
+  //      inside a word, and the three groups reduced over r3 become ORs across words ----
+  NodeFields F;
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+    F.base[a] = fields + 4 * int64_t(S.field_off[a]) + 60 * int64_t(8 * B.b[a] + loc[a]);
+  uint64_t A[3] = {0, 0, 0}, Bv[3] = {0, 0, 0}, Cv[3] = {0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    uint64_t x[3];
+    g2(F, k, 0, FH, 2, FH, x);
+    for (int t = 0; t < 3; t++) A[t] |= x[t];
+    g2(F, k, 0, FV, 1, FV, x);
+    for (int t = 0; t < 3; t++) Bv[t] |= x[t];
+    g3(F, k, 1, FLO, 0, FAS, 2, FLO, x);
+    for (int t = 0; t < 3; t++) Cv[t] |= x[t];
+    PE_FENCE();
+  }
+  S4 o0, o1;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    uint64_t w[3], x3[3], x4[3], x2[3], x5[3], x7[3];
+    g2(F, k, 2, FV, 3, FV, x3);
+    g3(F, k, 0, FLO, 2, FAS, 3, FLO, x4);
+    g2(F, k, 1, FH, 3, FH, x2);
+    g3(F, k, 0, FHI, 1, FAS, 3, FHI, x5);
+    g3(F, k, 1, FHI, 3, FAS, 2, FHI, x7);
+    for (int t = 0; t < 3; t++)
+      w[t] = red1(A[t] | red0(x3[t]) | x4[t]) | red2(Bv[t] | red0(x2[t]) | x5[t]) | Cv[t] | red0(x7[t]);
+    const uint64_t imm = w[0] | w[1];
+    o0.w[k] = ((w[2] | r1.w[k]) & ~imm) | (w[0] & ~w[1]);
+    o1.w[k] = (r0.w[k] & ~imm) | w[0];
+    PE_FENCE();
+  }
   uint64_t* o = output + 8 * (B.out + p);
-  st(o, s4_or(s4_andnot(wins, imm), s4_andnot(w0, w1)));
-  st(o + 4, s4_or(s4_andnot(nl, imm), w0));
+  st(o, o0);
+  st(o + 4, o1);
 }
 
 int64_t slab_index(const pe_slab& s, const int b[4]) {

@@ -66,6 +66,29 @@ PE_HD S4 shift_local(S4 a, int y) {
   a = shift2(a, y >> 4 & 3);
   return shift3(a, y >> 6 & 3);
 }
+// X'(z) = X(z + k e_q) for a per-thread amount k in 0..3 and a coordinate q that is the
+// same for every thread of a wave (branch-free in k).
+PE_HD S4 shift_coord_var(S4 a, int q, int k) {
+  k &= 3;
+  if (q == 0) {
+    const uint64_t m = k == 0 ? ~uint64_t(0) : k == 1 ? 0x7777777777777777ull : k == 2 ? 0x3333333333333333ull : 0x1111111111111111ull;
+    for (int i = 0; i < 4; i++) a.w[i] = ((a.w[i] >> k) & m) | ((a.w[i] << (4 - k)) & ~m);
+  } else if (q == 1) {
+    const uint64_t m = k == 0 ? ~uint64_t(0) : k == 1 ? 0x0FFF0FFF0FFF0FFFull : k == 2 ? 0x00FF00FF00FF00FFull : 0x000F000F000F000Full;
+    const int s = 4 * k;
+    for (int i = 0; i < 4; i++) a.w[i] = ((a.w[i] >> s) & m) | ((a.w[i] << (16 - s)) & ~m);
+  } else if (q == 2) {
+    const int s = 16 * k;
+    for (int i = 0; i < 4; i++) a.w[i] = (a.w[i] >> s) | (a.w[i] << ((64 - s) & 63));
+  } else {
+    S4 r = a;
+    if (k & 1) { r.w[0] = a.w[1]; r.w[1] = a.w[2]; r.w[2] = a.w[3]; r.w[3] = a.w[0]; }
+    S4 t = r;
+    if (k & 2) { t.w[0] = r.w[2]; t.w[1] = r.w[3]; t.w[2] = r.w[0]; t.w[3] = r.w[1]; }
+    return t;
+  }
+  return a;
+}
 PE_HD S4 shift_coord(S4 a, int q, int k) {
   switch (q) {
     case 0: return shift0(a, k);

@@ -17,5 +17,8 @@ void hc_super_wins(uint64_t side, const uint64_t* table, uint64_t* out) {
 void hc_split(const uint64_t* in, int parity, uint64_t* out) {
   S4 a; std::memcpy(a.w, in, 32); H2 h = split_half(a, parity); std::memcpy(out, h.w, 16);
 }
+void hc_shift_var(int q, int k, const uint64_t* in, uint64_t* out) {
+  S4 a; std::memcpy(a.w, in, 32); a = shift_coord_var(a, q, k); std::memcpy(out, a.w, 32);
+}
 void hc_hrmax(const uint64_t* in, uint64_t* out) { H2 h; std::memcpy(h.w, in, 16); h = hrmax(h); std::memcpy(out, h.w, 16); }
 }

@@ -53,9 +53,19 @@ def test_transform_all_symmetries(hc):
             assert val(out) == O.transform_super((g, y), s), (g, y)
 
 
+def test_shift_one_coordinate(hc):
+    """shift_coord_var(X, q, k)(z) = X(z + k e_q), the identity transform with y = k << 2q."""
+    rng = np.random.default_rng(5)
+    for q in range(4):
+        for k in range(4):
+            for _ in range(20):
+                s = int.from_bytes(rng.bytes(32), "little")
+                out = (C.c_uint64 * 4)()
+                hc.hc_shift_var(q, k, arr(s, 4), out)
+                assert val(out) == O.transform_super((0, k << (2 * q)), s), (q, k)
+
+
 def test_super_wins(hc):
-    table = np.ctypeslib.as_array(O.lib().po_superwin_info.__call__() if False else None) if False else None
-    info = C.cast(O.lib().po_superwin_info, C.c_void_p)
     f = O.lib().po_superwin_info
     f.restype = C.POINTER(C.c_uint64)
     tab = f()
