@@ -4,13 +4,16 @@ Workload (the largest one-GPU share of the end engine): a box of k^4 blocks (8x8
 nodes each) of the slice-24 section (3,3)(3,3)(3,3)(3,3), every open dimension computed.
 The box's child blocks (standard child section (4,3)(3,3)(3,3)(3,3): every block along the
 child dimension times the box cross-section) are resident in HBM as a dense slab filled with
-seeded data (known ~ Bernoulli(0.9) bits, win ~ Bernoulli(0.5) & known).  One step = the four
-dimension passes: each pass launches every block line crossing the box (restricted to the
-box along its dimension), reading child blocks from the slab and OR-merging into the box
-output.  Inputs are on the device before timing starts.
+seeded data (known ~ Bernoulli(0.9) bits, win ~ Bernoulli(0.5) & known).  One step = one
+pass of the end engine over the box: mode 0 (default) launches the fused block kernel once
+(every node gets all four dimensions and is written once); mode 1 launches four line passes,
+one per dimension, OR-merging into the box.  Inputs are on the device before timing starts.
 
 Prints one JSON line.  Metric: superpositions/s = box output nodes per second (the
 reference's output nodes/second, pentago/mpi/toplevel.cc:83-94).
+
+--copy-probe BYTES runs one plain device copy of BYTES bytes instead (a kernel of known
+traffic, used to scale the FETCH_SIZE / WRITE_SIZE counters).
 """
 import argparse
 import json
@@ -62,7 +65,7 @@ def cpu_baseline(budget_s=15.0):
     threaded one microline per job; superpositions/s = node-line rate / open dimensions."""
     from oracle import oracle as O
     from tests.seeded import seeded_nodes
-    threads = min(os.cpu_count() or 1, 128)
+    threads = min(os.cpu_count() or 1, 16)
     lines, ins, outs = [], [], []
     nodes = 0
     for d in range(4):
@@ -88,6 +91,19 @@ def cpu_baseline(budget_s=15.0):
                       "%d rounds in %.1f s; rate / 4 open dimensions" % (len(lines), nodes, rounds, dt)}
 
 
+def copy_probe(nbytes):
+    """One device-to-device copy of nbytes through torch's elementwise copy kernel."""
+    import torch
+    n = nbytes // 4
+    x = torch.ones(n, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    assert bool((y[-4:] == 1).all())
+    print(json.dumps({"copy_probe_bytes": 4 * n, "read": 4 * n, "written": 4 * n}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,11 +113,15 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", type=int, default=0, help="0: fused block kernel, 1: four line passes")
+    ap.add_argument("--copy-probe", type=int, default=0, help="run one plain copy of this many bytes instead")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU; --gpus %d is not supported" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     import pentago_amd as P
     from pentago_amd import hip
@@ -174,7 +194,7 @@ def main():
         "dtype": "u64",
         "data": "synthetic (seeded child blocks: known ~ Bernoulli(0.9), win ~ Bernoulli(0.5) & known)",
         "config": {"workload": "end-engine box of %d^4 blocks of section 24-(3,3)^4, child slab of section 25-%s, "
-                               "4 dimension passes" % (k, "".join("%d%d" % c for c in child)),
+                               "all 4 dimensions" % (k, "".join("%d%d" % c for c in child)),
                    "box_blocks": k ** 4, "output_nodes": n_out, "child_nodes_per_dimension": child_nodes, "mode": "fused" if args.mode == 0 else "line passes",
                    "lines_per_pass": k ** 3},
         "hbm_GBps_algorithmic": alg_step / (ms_per_step / 1e3) / 1e9,

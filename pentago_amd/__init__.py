@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpentago_amd.so")
+# PENTAGO_AMD_LIB selects another build of the library (same C-ABI), e.g. for A/B timing.
+LIB_PATH = os.environ.get("PENTAGO_AMD_LIB") or os.path.join(HERE, "libpentago_amd.so")
 
 
 class PentagoError(RuntimeError):

@@ -169,8 +169,10 @@ void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __r
     uint64_t x[3];
     g2(F, k, 0, FH, 2, FH, x);
     for (int t = 0; t < 3; t++) A[t] |= x[t];
+    PE_FENCE();
     g2(F, k, 0, FV, 1, FV, x);
     for (int t = 0; t < 3; t++) Bv[t] |= x[t];
+    PE_FENCE();
     g3(F, k, 1, FLO, 0, FAS, 2, FLO, x);
     for (int t = 0; t < 3; t++) Cv[t] |= x[t];
     PE_FENCE();
@@ -178,14 +180,17 @@ void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __r
   S4 o0, o1;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    uint64_t w[3], x3[3], x4[3], x2[3], x5[3], x7[3];
-    g2(F, k, 2, FV, 3, FV, x3);
-    g3(F, k, 0, FLO, 2, FAS, 3, FLO, x4);
-    g2(F, k, 1, FH, 3, FH, x2);
-    g3(F, k, 0, FHI, 1, FAS, 3, FHI, x5);
-    g3(F, k, 1, FHI, 3, FAS, 2, FHI, x7);
-    for (int t = 0; t < 3; t++)
-      w[t] = red1(A[t] | red0(x3[t]) | x4[t]) | red2(Bv[t] | red0(x2[t]) | x5[t]) | Cv[t] | red0(x7[t]);
+    uint64_t w[3], x[3], y[3];
+    g2(F, k, 2, FV, 3, FV, x);
+    g3(F, k, 0, FLO, 2, FAS, 3, FLO, y);
+    for (int t = 0; t < 3; t++) w[t] = red1(A[t] | red0(x[t]) | y[t]) | Cv[t];
+    PE_FENCE();
+    g2(F, k, 1, FH, 3, FH, x);
+    g3(F, k, 0, FHI, 1, FAS, 3, FHI, y);
+    for (int t = 0; t < 3; t++) w[t] |= red2(Bv[t] | red0(x[t]) | y[t]);
+    PE_FENCE();
+    g3(F, k, 1, FHI, 3, FAS, 2, FHI, x);
+    for (int t = 0; t < 3; t++) w[t] |= red0(x[t]);
     const uint64_t imm = w[0] | w[1];
     o0.w[k] = ((w[2] | r1.w[k]) & ~imm) | (w[0] & ~w[1]);
     o1.w[k] = (r0.w[k] & ~imm) | w[0];

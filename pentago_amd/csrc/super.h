@@ -55,15 +55,20 @@ PE_HD S4 shift0(S4 a, int k) { for (int i = 0; i < 4; i++) a.w[i] = shift0_word(
 PE_HD S4 shift1(S4 a, int k) { for (int i = 0; i < 4; i++) a.w[i] = shift1_word(a.w[i], k & 3); return a; }
 PE_HD S4 shift2(S4 a, int k) { for (int i = 0; i < 4; i++) a.w[i] = shift2_word(a.w[i], k & 3); return a; }
 PE_HD S4 shift3(S4 a, int k) {
-  S4 r;
-  for (int i = 0; i < 4; i++) r.w[i] = a.w[(i + k) & 3];
-  return r;
+  // word rotation by k as two masked selects (a per-thread index into the words would be
+  // lowered to a private-memory array)
+  const uint64_t m1 = uint64_t(0) - uint64_t(k & 1), m2 = uint64_t(0) - uint64_t(k >> 1 & 1);
+  S4 r, t;
+  for (int i = 0; i < 4; i++) r.w[i] = (a.w[i] & ~m1) | (a.w[(i + 1) & 3] & m1);
+  for (int i = 0; i < 4; i++) t.w[i] = (r.w[i] & ~m2) | (r.w[(i + 2) & 3] & m2);
+  return t;
 }
-// X'(z) = X(z + y) for a packed local rotation y (2 bits per quadrant).
+PE_HD S4 shift_coord_var(S4 a, int q, int k);
+// X'(z) = X(z + y) for a packed local rotation y (2 bits per quadrant), branch-free.
 PE_HD S4 shift_local(S4 a, int y) {
-  a = shift0(a, y & 3);
-  a = shift1(a, y >> 2 & 3);
-  a = shift2(a, y >> 4 & 3);
+  a = shift_coord_var(a, 0, y & 3);
+  a = shift_coord_var(a, 1, y >> 2 & 3);
+  a = shift_coord_var(a, 2, y >> 4 & 3);
   return shift3(a, y >> 6 & 3);
 }
 // X'(z) = X(z + k e_q) for a per-thread amount k in 0..3 and a coordinate q that is the
@@ -81,11 +86,7 @@ PE_HD S4 shift_coord_var(S4 a, int q, int k) {
     const int s = 16 * k;
     for (int i = 0; i < 4; i++) a.w[i] = (a.w[i] >> s) | (a.w[i] << ((64 - s) & 63));
   } else {
-    S4 r = a;
-    if (k & 1) { r.w[0] = a.w[1]; r.w[1] = a.w[2]; r.w[2] = a.w[3]; r.w[3] = a.w[0]; }
-    S4 t = r;
-    if (k & 2) { t.w[0] = r.w[2]; t.w[1] = r.w[3]; t.w[2] = r.w[0]; t.w[3] = r.w[1]; }
-    return t;
+    return shift3(a, k);
   }
   return a;
 }
