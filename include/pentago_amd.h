@@ -157,6 +157,10 @@ typedef struct { uint64_t win[2]; uint64_t notlose[2]; } pe_halfsupers;
 /* (raw board, value) pair of mid_values_t.  pentago/mid/midengine.h:36 */
 typedef struct { uint64_t raw; int32_t value; int32_t pad; } pe_mid_value;
 
+/* Whether the game is over and its value for the player to move (1 win, 0 tie, -1 loss).
+ * high_board_t::done_and_value, pentago/high/board.h:56-60. */
+pe_status pe_high_done(const pe_high_board* board, int* done, int* value);
+
 /* Workspace entries needed for boards with at least min_slice stones.
  * pentago/mid/midengine.cc:95-97 (entries are 32-byte (win, notlose) pairs here). */
 int64_t pe_midsolve_workspace_size(int min_slice);
@@ -174,6 +178,38 @@ pe_status pe_midsolve(const pe_high_board* board, pe_mid_value* values, int* cou
  * mode 0 = level-synchronous launches, 1 = persistent work-queue kernel. */
 pe_status pe_midsolve_batch(int n, const pe_high_board* boards, pe_halfsupers (*results)[19], int mode,
                             void* stream);
+
+/* Work-queue plans.  Every board of a batch (any mix of 6..18 empty cells) runs in one
+ * persistent launch whose workgroups pop tasks from a device queue: per board a slot gate,
+ * the table pass and one task group per level, each level waiting only for the same
+ * board's previous level.  Workspace is held inside an HBM budget: a board is admitted when
+ * a slot frees up.  No counterpart in the reference, which solves one board per call
+ * (pentago/mid/midengine.cc:57-125); each board's results equal pe_midsolve_internal's.
+ * Zero fields take the device defaults (set by pe_mid_autotune, else built in). */
+typedef struct {
+  int64_t workspace_budget;   /* bytes of workspace held at once (0: 60% of free HBM, <= 200 GB) */
+  int32_t tile;               /* minimum entries per task */
+  int32_t workgroups;         /* persistent workgroups of 256 threads */
+  int64_t wait_limit_us;      /* bound on one dependency wait; exceeded -> PE_ERR_DEVICE (0: 10 s) */
+  int32_t use_graph;          /* nonzero: pe_mid_plan_run replays a captured hipGraph */
+  int32_t pad;
+} pe_mid_config;
+typedef struct pe_mid_plan pe_mid_plan;
+pe_status pe_mid_plan_create(int n, const pe_high_board* boards, const pe_mid_config* config, pe_mid_plan** plan);
+/* Asynchronous on `stream`; a plan may be run any number of times (not concurrently). */
+pe_status pe_mid_plan_run(pe_mid_plan* plan, void* stream);
+/* Waits for the last run on `stream` and copies results[i][0..18] (entries past spots zero). */
+pe_status pe_mid_plan_results(pe_mid_plan* plan, pe_halfsupers (*results)[19], void* stream);
+/* Tasks in the queue, workspace bytes held, and the configuration in effect. */
+pe_status pe_mid_plan_info(const pe_mid_plan* plan, int64_t* tasks, int64_t* workspace_bytes, pe_mid_config* config);
+pe_status pe_mid_plan_destroy(pe_mid_plan* plan);
+/* Host-only: build and check the task order and workspace slots a plan would use (no
+ * device calls; every config field must be set).  Returns the task count and workspace. */
+pe_status pe_mid_schedule_info(int n, const pe_high_board* boards, const pe_mid_config* config, int64_t* tasks,
+                               int64_t* workspace_bytes);
+/* Time candidate (tile, workgroups) pairs on these boards and keep the fastest as the
+ * current device's default.  The chosen configuration is returned through *best. */
+pe_status pe_mid_autotune(int n, const pe_high_board* boards, int64_t workspace_budget, pe_mid_config* best);
 
 /* Traverse precomputed results into (raw, value) pairs.  pentago/mid/midengine.cc:175-206 */
 pe_status pe_midsolve_traverse(const pe_high_board* board, const pe_halfsupers* results,

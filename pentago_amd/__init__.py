@@ -89,6 +89,17 @@ class HalfSupers(C.Structure):
     _fields_ = [("win", C.c_uint64 * 2), ("notlose", C.c_uint64 * 2)]
 
 
+class MidConfig(C.Structure):
+    """pe_mid_config: zero fields take the device defaults."""
+    _fields_ = [("workspace_budget", C.c_int64), ("tile", C.c_int32), ("workgroups", C.c_int32),
+                ("wait_limit_us", C.c_int64), ("use_graph", C.c_int32), ("pad", C.c_int32)]
+
+    def as_dict(self):
+        return {"workspace_budget": int(self.workspace_budget), "tile": int(self.tile),
+                "workgroups": int(self.workgroups), "wait_limit_us": int(self.wait_limit_us),
+                "use_graph": bool(self.use_graph)}
+
+
 class MidValue(C.Structure):
     _fields_ = [("raw", C.c_uint64), ("value", C.c_int32), ("pad", C.c_int32)]
 
@@ -129,6 +140,14 @@ def lib():
             "pe_midsolve": (C.c_int, [P(HighBoard), P(MidValue), P(C.c_int)]),
             "pe_midsolve_batch": (C.c_int, [C.c_int, P(HighBoard), V, C.c_int, V]),
             "pe_midsolve_traverse": (C.c_int, [P(HighBoard), P(HalfSupers), P(MidValue), P(C.c_int)]),
+            "pe_high_done": (C.c_int, [P(HighBoard), P(C.c_int), P(C.c_int)]),
+            "pe_mid_plan_create": (C.c_int, [C.c_int, P(HighBoard), P(MidConfig), P(V)]),
+            "pe_mid_plan_run": (C.c_int, [V, V]),
+            "pe_mid_plan_results": (C.c_int, [V, V, V]),
+            "pe_mid_plan_info": (C.c_int, [V, P(C.c_int64), P(C.c_int64), P(MidConfig)]),
+            "pe_mid_plan_destroy": (C.c_int, [V]),
+            "pe_mid_schedule_info": (C.c_int, [C.c_int, P(HighBoard), P(MidConfig), P(C.c_int64), P(C.c_int64)]),
+            "pe_mid_autotune": (C.c_int, [C.c_int, P(HighBoard), C.c_int64, P(MidConfig)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -297,6 +316,13 @@ def high_from_board(board, middle):
     return high_board(s0, s1, 2 * bin(s0 | s1).count("1") - int(bool(middle)))
 
 
+def high_done(board):
+    """high_board_t::done_and_value (pentago/high/board.h:56): (done, value)."""
+    done, value = C.c_int(), C.c_int()
+    _check(lib().pe_high_done(C.byref(board), C.byref(done), C.byref(value)), "pe_high_done")
+    return bool(done.value), int(value.value)
+
+
 def _halves(r):
     return (int(r.win[0]) | int(r.win[1]) << 64, int(r.notlose[0]) | int(r.notlose[1]) << 64)
 
@@ -318,6 +344,7 @@ def midsolve(board):
 
 
 def midsolve_batch(boards, mode=0, stream=None):
+    """pe_midsolve_batch: mode 0 level-synchronous launches, 1 persistent work queue."""
     n = len(boards)
     arr = (HighBoard * n)(*boards)
     res = ((HalfSupers * 19) * n)()
@@ -327,3 +354,79 @@ def midsolve_batch(boards, mode=0, stream=None):
         spots = 36 - ((boards[i].ply + 1) >> 1)
         out.append([_halves(res[i][k]) for k in range(spots + 1)])
     return out
+
+
+def _spots(board):
+    return 36 - ((board.ply + 1) >> 1)
+
+
+def results_to_lists(boards, res):
+    """(n, 19, 4) uint64 results -> per board the list of (win, notlose) halfsuper ints."""
+    out = []
+    for i, b in enumerate(boards):
+        r = res[i]
+        out.append([(int(r[k, 0]) | int(r[k, 1]) << 64, int(r[k, 2]) | int(r[k, 3]) << 64)
+                    for k in range(_spots(b) + 1)])
+    return out
+
+
+class MidPlan:
+    """A batch of boards solved by the persistent work-queue kernel (pe_mid_plan_*).
+
+    Build once, run any number of times; results() waits for the last run and returns an
+    (n, 19, 4) uint64 array (win lo, win hi, notlose lo, notlose hi per entry)."""
+
+    def __init__(self, boards, workspace_budget=0, tile=0, workgroups=0, wait_limit_us=0, use_graph=False):
+        self.boards = list(boards)
+        n = len(self.boards)
+        arr = (HighBoard * max(n, 1))(*self.boards)
+        cfg = MidConfig(workspace_budget, tile, workgroups, wait_limit_us, int(bool(use_graph)), 0)
+        self.ptr = C.c_void_p()
+        _check(lib().pe_mid_plan_create(n, arr, C.byref(cfg), C.byref(self.ptr)), "pe_mid_plan_create")
+
+    def run(self, stream=None):
+        _check(lib().pe_mid_plan_run(self.ptr, stream), "pe_mid_plan_run")
+
+    def results(self, stream=None):
+        import numpy as np
+        res = np.zeros((max(len(self.boards), 1), 19, 4), dtype=np.uint64)
+        _check(lib().pe_mid_plan_results(self.ptr, res.ctypes.data_as(C.c_void_p), stream), "pe_mid_plan_results")
+        return res[:len(self.boards)]
+
+    def info(self):
+        tasks, ws, cfg = C.c_int64(), C.c_int64(), MidConfig()
+        _check(lib().pe_mid_plan_info(self.ptr, C.byref(tasks), C.byref(ws), C.byref(cfg)), "pe_mid_plan_info")
+        d = cfg.as_dict()
+        d.update(tasks=int(tasks.value), workspace_bytes=int(ws.value))
+        return d
+
+    def close(self):
+        if self.ptr:
+            lib().pe_mid_plan_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mid_schedule_info(boards, workspace_budget, tile, workgroups):
+    """pe_mid_schedule_info: (tasks, workspace bytes) of the schedule a plan would use."""
+    n = len(boards)
+    arr = (HighBoard * max(n, 1))(*boards)
+    cfg = MidConfig(workspace_budget, tile, workgroups, 0, 0, 0)
+    tasks, ws = C.c_int64(), C.c_int64()
+    _check(lib().pe_mid_schedule_info(n, arr, C.byref(cfg), C.byref(tasks), C.byref(ws)), "pe_mid_schedule_info")
+    return int(tasks.value), int(ws.value)
+
+
+def mid_autotune(boards, workspace_budget=0):
+    """pe_mid_autotune: time candidate (tile, workgroups) pairs on these boards; the best
+    becomes the device default.  Returns the chosen configuration."""
+    n = len(boards)
+    arr = (HighBoard * n)(*boards)
+    best = MidConfig()
+    _check(lib().pe_mid_autotune(n, arr, workspace_budget, C.byref(best)), "pe_mid_autotune")
+    return best.as_dict()

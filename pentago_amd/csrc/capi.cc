@@ -10,6 +10,7 @@
 #include "device.h"
 #include "end.h"
 #include "mid.h"
+#include "mid_queue.h"
 #include "tables.h"
 
 namespace pe {
@@ -26,6 +27,7 @@ pe_status launch_fill_seeded(void* d_nodes, int64_t nodes, uint64_t seed, int64_
 size_t mid_board_bytes(const MidInfo& I);
 pe_status mid_solve_levels(const std::vector<MidInfo>& infos, uint8_t* arena, std::vector<size_t>* result_offsets,
                            hipStream_t stream);
+int mid_queue_blocks_per_cu();
 
 namespace {
 thread_local std::string g_error;
@@ -317,10 +319,199 @@ int64_t pe_midsolve_workspace_size(int min_slice) {
   return bottleneck(36 - min_slice);
 }
 
+}  // extern "C"
+
+namespace {
+
+struct TunedMid { bool set = false; int32_t tile = 0, workgroups = 0; };
+TunedMid g_tuned[kMaxDevices];
+
+pe_status mid_infos(int n, const pe_high_board* boards, std::vector<MidInfo>* infos) {
+  infos->resize(n);
+  for (int b = 0; b < n; b++)
+    if (!make_mid_info(boards[b], &(*infos)[b]))
+      return fail(PE_ERR_INVALID, "board " + std::to_string(b) + " is not a valid board with >= 18 stones");
+  return PE_OK;
+}
+
+// Fill the zero fields of a configuration with the current device's defaults.
+pe_status mid_resolve_config(const pe_mid_config* in, MidConfig* out) {
+  int dev = 0;
+  PE_HIP(hipGetDevice(&dev), "hipGetDevice");
+  int cus = 0, rate_khz = 0;
+  PE_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute(CUs)");
+  PE_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev), "hipDeviceGetAttribute(wall clock)");
+  pe_mid_config c{};
+  if (in) c = *in;
+  if (c.workspace_budget <= 0) {
+    size_t free_b = 0, total_b = 0;
+    PE_HIP(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    c.workspace_budget = std::min<int64_t>(int64_t(200e9), int64_t(0.6 * double(free_b)));
+  }
+  const TunedMid& t = g_tuned[dev];
+  if (c.tile <= 0) c.tile = t.set ? t.tile : 4096;
+  if (c.workgroups <= 0) c.workgroups = t.set ? t.workgroups : cus * std::max(1, mid_queue_blocks_per_cu());
+  if (c.wait_limit_us <= 0) c.wait_limit_us = 10 * 1000 * 1000;
+  out->budget = c.workspace_budget;
+  out->tile = c.tile;
+  out->workgroups = c.workgroups;
+  out->wait_limit = c.wait_limit_us * int64_t(std::max(rate_khz, 1)) / 1000;
+  out->graph = c.use_graph != 0;
+  return PE_OK;
+}
+
+}  // namespace
+
+struct pe_mid_plan {
+  MidPlan plan;
+  int64_t wait_limit_us;
+};
+
+extern "C" {
+
+pe_status pe_mid_plan_create(int n, const pe_high_board* boards, const pe_mid_config* config, pe_mid_plan** plan) {
+  if (n < 0 || (n && !boards) || !plan) return fail(PE_ERR_INVALID, "bad plan arguments");
+  *plan = nullptr;
+  std::vector<MidInfo> infos;
+  pe_status st = mid_infos(n, boards, &infos);
+  if (st != PE_OK) return st;
+  MidConfig cfg;
+  st = mid_resolve_config(config, &cfg);
+  if (st != PE_OK) return st;
+  auto* p = new pe_mid_plan;
+  p->wait_limit_us = config && config->wait_limit_us > 0 ? config->wait_limit_us : 10 * 1000 * 1000;
+  st = mid_plan_create(infos, cfg, &p->plan);
+  if (st != PE_OK) { delete p; return st; }
+  *plan = p;
+  return PE_OK;
+}
+
+pe_status pe_mid_schedule_info(int n, const pe_high_board* boards, const pe_mid_config* config, int64_t* tasks,
+                               int64_t* workspace_bytes) {
+  if (n < 0 || (n && !boards) || !config || config->workspace_budget <= 0 || config->tile <= 0 ||
+      config->workgroups <= 0)
+    return fail(PE_ERR_INVALID, "schedule info needs a board list and a complete configuration");
+  std::vector<MidInfo> infos;
+  pe_status st = mid_infos(n, boards, &infos);
+  if (st != PE_OK) return st;
+  MidConfig cfg;
+  cfg.budget = config->workspace_budget;
+  cfg.tile = config->tile;
+  cfg.workgroups = config->workgroups;
+  cfg.wait_limit = 0;
+  cfg.graph = false;
+  std::vector<MidTask> order;
+  std::vector<int2> deps;
+  std::vector<int64_t> offsets;
+  int64_t arena = 0;
+  int32_t nslots = 0;
+  if (const char* e = mid_plan_build_schedule(infos, cfg, &order, &deps, &offsets, &arena, &nslots))
+    return fail(std::strncmp(e, "internal", 8) ? PE_ERR_CAPACITY : PE_ERR_INVALID, e);
+  if (tasks) *tasks = int64_t(order.size());
+  if (workspace_bytes) *workspace_bytes = arena;
+  return PE_OK;
+}
+
+pe_status pe_mid_plan_run(pe_mid_plan* plan, void* stream) {
+  if (!plan) return fail(PE_ERR_INVALID, "null plan");
+  return mid_plan_run(plan->plan, static_cast<hipStream_t>(stream));
+}
+
+pe_status pe_mid_plan_results(pe_mid_plan* plan, pe_halfsupers (*results)[19], void* stream) {
+  if (!plan || (plan->plan.nb && !results)) return fail(PE_ERR_INVALID, "null argument");
+  return mid_plan_results(plan->plan, results, static_cast<hipStream_t>(stream));
+}
+
+pe_status pe_mid_plan_info(const pe_mid_plan* plan, int64_t* tasks, int64_t* workspace_bytes, pe_mid_config* config) {
+  if (!plan) return fail(PE_ERR_INVALID, "null plan");
+  if (tasks) *tasks = plan->plan.ntasks;
+  if (workspace_bytes) *workspace_bytes = plan->plan.arena_bytes;
+  if (config) {
+    std::memset(config, 0, sizeof *config);
+    config->workspace_budget = plan->plan.cfg.budget;
+    config->tile = plan->plan.cfg.tile;
+    config->workgroups = plan->plan.cfg.workgroups;
+    config->wait_limit_us = plan->wait_limit_us;
+    config->use_graph = plan->plan.cfg.graph;
+  }
+  return PE_OK;
+}
+
+pe_status pe_mid_plan_destroy(pe_mid_plan* plan) {
+  if (!plan) return PE_OK;
+  mid_plan_destroy(&plan->plan);
+  delete plan;
+  return PE_OK;
+}
+
+pe_status pe_mid_autotune(int n, const pe_high_board* boards, int64_t workspace_budget, pe_mid_config* best) {
+  if (n <= 0 || !boards || !best) return fail(PE_ERR_INVALID, "bad autotune arguments");
+  std::vector<MidInfo> infos;
+  pe_status st = mid_infos(n, boards, &infos);
+  if (st != PE_OK) return st;
+  int dev = 0, cus = 0;
+  PE_HIP(hipGetDevice(&dev), "hipGetDevice");
+  PE_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute(CUs)");
+  const int per_cu = std::max(1, mid_queue_blocks_per_cu());
+  const int32_t tiles[] = {1024, 4096, 16384};
+  const int32_t wgs[] = {cus * std::max(1, per_cu / 2), cus * per_cu, cus * per_cu * 2};
+  hipStream_t s = nullptr;
+  PE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  float best_ms = -1;
+  pe_mid_config chosen{};
+  for (int32_t tile : tiles)
+    for (int32_t wg : wgs) {
+      pe_mid_config c{};
+      c.workspace_budget = workspace_budget;
+      c.tile = tile;
+      c.workgroups = wg;
+      MidConfig cfg;
+      st = mid_resolve_config(&c, &cfg);
+      if (st != PE_OK) break;
+      MidPlan plan;
+      st = mid_plan_create(infos, cfg, &plan);
+      if (st != PE_OK) break;
+      float ms = 0;
+      // one warm run, one timed run
+      st = mid_plan_run(plan, s);
+      if (st == PE_OK && hipEventRecord(e0, s) == hipSuccess) st = mid_plan_run(plan, s);
+      if (st == PE_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                          hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        st = fail(PE_ERR_DEVICE, "autotune timing failed");
+      int32_t err = 0;
+      if (st == PE_OK && hipMemcpy(&err, plan.error, sizeof err, hipMemcpyDeviceToHost) == hipSuccess && err)
+        st = fail(PE_ERR_DEVICE, "autotune: dependency wait exceeded the limit");
+      mid_plan_destroy(&plan);
+      if (st != PE_OK) break;
+      if (best_ms < 0 || ms < best_ms) { best_ms = ms; chosen = c; }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+  if (st != PE_OK) return st;
+  g_tuned[dev].set = true;
+  g_tuned[dev].tile = chosen.tile;
+  g_tuned[dev].workgroups = chosen.workgroups;
+  *best = chosen;
+  return PE_OK;
+}
+
 pe_status pe_midsolve_batch(int n, const pe_high_board* boards, pe_halfsupers (*results)[19], int mode,
                             void* stream) {
   if (n < 0 || (n && (!boards || !results))) return fail(PE_ERR_INVALID, "bad batch");
-  if (mode != 0) return fail(PE_ERR_INVALID, "unsupported batch mode");
+  if (mode != 0 && mode != 1) return fail(PE_ERR_INVALID, "unsupported batch mode");
+  if (mode == 1) {
+    pe_mid_plan* plan = nullptr;
+    pe_status st = pe_mid_plan_create(n, boards, nullptr, &plan);
+    if (st != PE_OK) return st;
+    st = pe_mid_plan_run(plan, stream);
+    if (st == PE_OK) st = pe_mid_plan_results(plan, results, stream);
+    pe_mid_plan_destroy(plan);
+    return st;
+  }
   std::vector<MidInfo> infos(n);
   size_t bytes = 0;
   for (int b = 0; b < n; b++) {
@@ -351,6 +542,13 @@ pe_status pe_midsolve_internal(const pe_high_board* board, pe_halfsupers results
   if (!board || !results) return fail(PE_ERR_INVALID, "null argument");
   pe_halfsupers (*r)[19] = reinterpret_cast<pe_halfsupers (*)[19]>(results);
   return pe_midsolve_batch(1, board, r, 0, nullptr);
+}
+
+pe_status pe_high_done(const pe_high_board* board, int* done, int* value) {
+  if (!board || !done || !value) return fail(PE_ERR_INVALID, "null argument");
+  if ((board->side[0] & board->side[1]) || board->ply > 72) return fail(PE_ERR_INVALID, "invalid board");
+  *done = high_done_and_value(*board, value) ? 1 : 0;
+  return PE_OK;
 }
 
 pe_status pe_midsolve_traverse(const pe_high_board* board, const pe_halfsupers* results, pe_mid_value* values,
