@@ -104,6 +104,136 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": 4 * n, "read": 4 * n, "written": 4 * n}), flush=True)
 
 
+def mid_bench(P, hip, n_boards=4096, seed=2):
+    """Batched midsolve: n seeded boards with 18..30 stones in one work-queue launch
+    (autotuned tile and workgroups) against one plain pe_midsolve_internal call per board;
+    every batched result is compared with the plain one.  Also the 26-stone single board."""
+    from pentago_amd.workloads import board_spots, seeded_boards
+    out = {}
+    # single board, 13 + 13 stones (plain per-board launch)
+    one = seeded_boards(1, 26, 26, 26)[0]
+    P.midsolve_internal(one)
+    lat = []
+    for _ in range(20):
+        t0 = time.perf_counter()
+        P.midsolve_internal(one)
+        lat.append(1e3 * (time.perf_counter() - t0))
+    out["single_26_stones_ms"] = float(np.median(lat))
+    boards = seeded_boards(n_boards, seed, 18, 30)
+    sample = boards[:512]
+    t0 = time.time()
+    tuned = P.mid_autotune(sample)
+    out["autotune"] = dict(tuned, seconds=time.time() - t0, boards=len(sample))
+    log("midsolve autotune: %s" % out["autotune"])
+    t0 = time.time()
+    plan = P.MidPlan(boards)
+    info = plan.info()
+    log("midsolve plan: %s, built in %.1f s" % (info, time.time() - t0))
+    stream = hip.Stream()
+    plan.run(stream.ptr)                       # warm-up
+    res = plan.results(stream.ptr)
+    ev0, ev1 = hip.Event(), hip.Event()
+    reps = 2
+    ms = []
+    for _ in range(reps):
+        ev0.record(stream)
+        plan.run(stream.ptr)
+        ev1.record(stream)
+        ev1.synchronize()
+        ms.append(ev0.elapsed_ms(ev1))
+    res2 = plan.results(stream.ptr)
+    plan.close()
+    batched_s = min(ms) / 1e3
+    log("midsolve batched: %s ms" % ms)
+    # plain: one synchronous launch sequence per board
+    plain = np.zeros_like(res)
+    t0 = time.perf_counter()
+    for i, b in enumerate(boards):
+        P.midsolve_internal_raw(b, plain[i])
+    plain_s = time.perf_counter() - t0
+    log("midsolve plain: %.2f s" % plain_s)
+    spots = np.array([board_spots(b) for b in boards])
+    mism = int(np.sum(np.any(plain != res, axis=(1, 2)))) + int(np.sum(np.any(res2 != res, axis=(1, 2))))
+    work = 0
+    from math import comb
+    for sp in spots:
+        sp = int(sp)
+        work += 2 * 16 * sum(comb(sp, n // 2) * comb(sp - n // 2, n - n // 2) * (sp - n + 1) for n in range(sp + 1))
+    out.update({
+        "config": {"workload": "%d seeded boards, 18..30 stones (6..18 empty cells)" % n_boards,
+                   "spots_histogram": {int(k): int(v) for k, v in zip(*np.unique(spots, return_counts=True))}},
+        "batched_boards_per_s": n_boards / batched_s, "batched_ms": min(ms), "batched_ms_runs": ms,
+        "plain_boards_per_s": n_boards / plain_s, "plain_s": plain_s,
+        "batched_over_plain": plain_s / batched_s,
+        "mismatches": mism, "plan": info,
+        "algorithmic_GBps": work / batched_s / 1e9,
+    })
+    return out
+
+
+def serve_bench(P, rates=(200, 1000, 5000), seed=3, seconds=2.0):
+    """Serving loop under Poisson arrivals: p50/p99 latency and solves/s per rate; every
+    answer compared with a work-queue batch solve of the same boards."""
+    import threading
+    from pentago_amd.workloads import seeded_boards
+    out = {}
+    for rate in rates:
+        n = int(min(4000, max(400, rate * seconds)))
+        boards = seeded_boards(n, seed + rate, 18, 34)
+        rng = np.random.default_rng(seed * 7 + rate)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n))
+        server = P.MidServer()
+        tickets = [None] * n
+        t_start = time.perf_counter()
+
+        def feed():
+            for i, b in enumerate(boards):
+                delay = arrivals[i] - (time.perf_counter() - t_start)
+                if delay > 0:
+                    time.sleep(delay)
+                tickets[i] = server.submit(b)
+
+        th = threading.Thread(target=feed)
+        th.start()
+        th.join()
+        got, lat = [], []
+        for t in tickets:
+            r, l = server.wait(t)
+            got.append(r)
+            lat.append(l)
+        wall = time.perf_counter() - t_start
+        stats = server.stats()
+        server.close()
+        plan = P.MidPlan(boards)
+        plan.run()
+        ref = P.results_to_lists(boards, plan.results())
+        plan.close()
+        mism = sum(1 for a, b in zip(got, ref) if a != b)
+        lat = np.array(lat)
+        out[str(rate)] = {"requests": n, "p50_ms": float(np.percentile(lat, 50)),
+                          "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
+                          "solves_per_s": n / wall, "offered_per_s": n / float(arrivals[-1]),
+                          "batches": stats["batches"], "peak_workspace_GB": stats["peak_workspace"] / 1e9,
+                          "mismatches": mism}
+        log("serving %d req/s: %s" % (rate, out[str(rate)]))
+    return out
+
+
+def mid_cpu_baseline(n=48, seed=2):
+    """The oracle's midsolve (one board per thread) on a sample of the batched workload."""
+    from oracle import oracle as O
+    from pentago_amd.workloads import seeded_boards
+    threads = min(os.cpu_count() or 1, 16)
+    boards = seeded_boards(4096, seed, 18, 30)[:n]
+    roots = [O.High((O.C.c_uint64 * 2)(b.side[0], b.side[1]), b.ply) for b in boards]
+    t0 = time.time()
+    O.midsolve_batch(roots, threads)
+    dt = time.time() - t0
+    return {"value": n / dt, "unit": "boards/s", "cores": threads, "kind": "port",
+            "sample": "first %d boards of the batched workload (18..30 stones), one board per thread, %.1f s"
+                      % (n, dt)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +244,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", type=int, default=0, help="0: fused block kernel, 1: four line passes")
     ap.add_argument("--copy-probe", type=int, default=0, help="run one plain copy of this many bytes instead")
+    ap.add_argument("--no-mid", action="store_true", help="skip the batched midsolve measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU; --gpus %d is not supported" % args.gpus,
@@ -207,11 +339,20 @@ def main():
     }
     plan.close()
     del out, kids
+    if not args.no_mid:
+        result["midsolve"] = mid_bench(P, hip)
+    if not args.no_serve:
+        result["serving"] = serve_bench(P)
     if not args.no_cpu:
         try:
             result["cpu_baseline"] = cpu_baseline(args.cpu_budget)
         except Exception as e:   # the baseline is reported, never required
             result["cpu_baseline"] = {"error": str(e)}
+        if not args.no_mid:
+            try:
+                result["midsolve"]["cpu_baseline"] = mid_cpu_baseline()
+            except Exception as e:
+                result["midsolve"]["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(result), flush=True)
 
 

@@ -211,6 +211,39 @@ pe_status pe_mid_schedule_info(int n, const pe_high_board* boards, const pe_mid_
  * current device's default.  The chosen configuration is returned through *best. */
 pe_status pe_mid_autotune(int n, const pe_high_board* boards, int64_t workspace_budget, pe_mid_config* best);
 
+/* Serving loop: single-position solve requests admitted as they arrive.  A worker thread
+ * batches the pending requests whose workspace fits the free part of an HBM pool and
+ * launches each batch on one of several streams through the work-queue kernel, so requests
+ * that arrive while others are on the device start as soon as memory and a stream free up;
+ * batches overlap on the device.  Results are those of pe_midsolve_internal.  The
+ * reference serves one wasm midsolve per request (web/server/index.js:16-58); this has no
+ * direct counterpart.  All calls are thread-safe; each ticket is taken by exactly one
+ * successful poll (done = 1) or wait. */
+typedef struct pe_mid_server pe_mid_server;
+typedef struct {
+  int64_t workspace_budget;   /* bytes of the workspace pool (0: 40% of free HBM, <= 100 GB) */
+  int32_t streams;            /* batches in flight at once (0: 4) */
+  int32_t max_batch;          /* requests per batch (0: 512) */
+  int32_t tile;               /* work-queue tile (0: 1024) */
+  int32_t workgroups;         /* work-queue workgroups per batch (0: one full device) */
+} pe_mid_server_config;
+typedef struct {
+  int64_t submitted, completed, batches, peak_workspace, pending, in_flight;
+} pe_mid_server_info;
+pe_status pe_mid_server_create(const pe_mid_server_config* config, pe_mid_server** server);
+/* Returns PE_ERR_INVALID for a board that is not solvable here, PE_ERR_CAPACITY when its
+ * workspace exceeds the whole pool. */
+pe_status pe_mid_server_submit(pe_mid_server* server, const pe_high_board* board, int64_t* ticket);
+/* Nonblocking.  *done = 1 once finished: results[0..18] and the latency (submit to
+ * completion, ms) are written and the ticket is released. */
+pe_status pe_mid_server_poll(pe_mid_server* server, int64_t ticket, pe_halfsupers results[19], int* done,
+                             double* latency_ms);
+/* Blocks until the request has finished, then as poll. */
+pe_status pe_mid_server_wait(pe_mid_server* server, int64_t ticket, pe_halfsupers results[19], double* latency_ms);
+pe_status pe_mid_server_stats(pe_mid_server* server, pe_mid_server_info* stats);
+/* Finishes every submitted request, then releases the server. */
+pe_status pe_mid_server_destroy(pe_mid_server* server);
+
 /* Traverse precomputed results into (raw, value) pairs.  pentago/mid/midengine.cc:175-206 */
 pe_status pe_midsolve_traverse(const pe_high_board* board, const pe_halfsupers* results,
                                pe_mid_value* values, int* count);

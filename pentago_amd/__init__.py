@@ -100,6 +100,16 @@ class MidConfig(C.Structure):
                 "use_graph": bool(self.use_graph)}
 
 
+class MidServerConfig(C.Structure):
+    _fields_ = [("workspace_budget", C.c_int64), ("streams", C.c_int32), ("max_batch", C.c_int32),
+                ("tile", C.c_int32), ("workgroups", C.c_int32)]
+
+
+class MidServerInfo(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("submitted", "completed", "batches", "peak_workspace", "pending",
+                                         "in_flight")]
+
+
 class MidValue(C.Structure):
     _fields_ = [("raw", C.c_uint64), ("value", C.c_int32), ("pad", C.c_int32)]
 
@@ -141,6 +151,12 @@ def lib():
             "pe_midsolve_batch": (C.c_int, [C.c_int, P(HighBoard), V, C.c_int, V]),
             "pe_midsolve_traverse": (C.c_int, [P(HighBoard), P(HalfSupers), P(MidValue), P(C.c_int)]),
             "pe_high_done": (C.c_int, [P(HighBoard), P(C.c_int), P(C.c_int)]),
+            "pe_mid_server_create": (C.c_int, [P(MidServerConfig), P(V)]),
+            "pe_mid_server_submit": (C.c_int, [V, P(HighBoard), P(C.c_int64)]),
+            "pe_mid_server_poll": (C.c_int, [V, C.c_int64, V, P(C.c_int), P(C.c_double)]),
+            "pe_mid_server_wait": (C.c_int, [V, C.c_int64, V, P(C.c_double)]),
+            "pe_mid_server_stats": (C.c_int, [V, P(MidServerInfo)]),
+            "pe_mid_server_destroy": (C.c_int, [V]),
             "pe_mid_plan_create": (C.c_int, [C.c_int, P(HighBoard), P(MidConfig), P(V)]),
             "pe_mid_plan_run": (C.c_int, [V, V]),
             "pe_mid_plan_results": (C.c_int, [V, V, V]),
@@ -335,6 +351,12 @@ def midsolve_internal(board):
     return [_halves(res[i]) for i in range(spots + 1)]
 
 
+def midsolve_internal_raw(board, out):
+    """midsolve_internal into a (19, 4) uint64 numpy row (win lo/hi, notlose lo/hi)."""
+    _check(lib().pe_midsolve_internal(C.byref(board), out.ctypes.data_as(C.POINTER(HalfSupers))),
+           "pe_midsolve_internal")
+
+
 def midsolve(board):
     """midsolve (pentago/mid/midengine.h:45): {raw board: value}."""
     vals = (MidValue * 163)()
@@ -430,3 +452,55 @@ def mid_autotune(boards, workspace_budget=0):
     best = MidConfig()
     _check(lib().pe_mid_autotune(n, arr, workspace_budget, C.byref(best)), "pe_mid_autotune")
     return best.as_dict()
+
+
+class MidServer:
+    """Serving loop (pe_mid_server_*): submit boards as they arrive, collect results.
+
+    submit() returns a ticket; wait(ticket) / poll(ticket) return (results, latency_ms) with
+    results the list of (win, notlose) halfsuper ints as midsolve_internal gives."""
+
+    def __init__(self, workspace_budget=0, streams=0, max_batch=0, tile=0, workgroups=0):
+        cfg = MidServerConfig(workspace_budget, streams, max_batch, tile, workgroups)
+        self.ptr = C.c_void_p()
+        self.spots = {}
+        _check(lib().pe_mid_server_create(C.byref(cfg), C.byref(self.ptr)), "pe_mid_server_create")
+
+    def submit(self, board):
+        t = C.c_int64()
+        _check(lib().pe_mid_server_submit(self.ptr, C.byref(board), C.byref(t)), "pe_mid_server_submit")
+        self.spots[t.value] = _spots(board)
+        return t.value
+
+    def _out(self, ticket, res, lat):
+        sp = self.spots.pop(ticket)
+        return [_halves(res[i]) for i in range(sp + 1)], lat.value
+
+    def wait(self, ticket):
+        res = (HalfSupers * 19)()
+        lat = C.c_double()
+        _check(lib().pe_mid_server_wait(self.ptr, ticket, res, C.byref(lat)), "pe_mid_server_wait")
+        return self._out(ticket, res, lat)
+
+    def poll(self, ticket):
+        res = (HalfSupers * 19)()
+        lat = C.c_double()
+        done = C.c_int()
+        _check(lib().pe_mid_server_poll(self.ptr, ticket, res, C.byref(done), C.byref(lat)), "pe_mid_server_poll")
+        return self._out(ticket, res, lat) if done.value else None
+
+    def stats(self):
+        info = MidServerInfo()
+        _check(lib().pe_mid_server_stats(self.ptr, C.byref(info)), "pe_mid_server_stats")
+        return {n: int(getattr(info, n)) for n, _ in MidServerInfo._fields_}
+
+    def close(self):
+        if self.ptr:
+            lib().pe_mid_server_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -405,7 +405,7 @@ pe_status pe_mid_schedule_info(int n, const pe_high_board* boards, const pe_mid_
   std::vector<int64_t> offsets;
   int64_t arena = 0;
   int32_t nslots = 0;
-  if (const char* e = mid_plan_build_schedule(infos, cfg, &order, &deps, &offsets, &arena, &nslots))
+  if (const char* e = mid_plan_build_schedule(infos, cfg, nullptr, &order, &deps, &offsets, &arena, &nslots))
     return fail(std::strncmp(e, "internal", 8) ? PE_ERR_CAPACITY : PE_ERR_INVALID, e);
   if (tasks) *tasks = int64_t(order.size());
   if (workspace_bytes) *workspace_bytes = arena;
@@ -496,6 +496,41 @@ pe_status pe_mid_autotune(int n, const pe_high_board* boards, int64_t workspace_
   g_tuned[dev].tile = chosen.tile;
   g_tuned[dev].workgroups = chosen.workgroups;
   *best = chosen;
+  return PE_OK;
+}
+
+pe_status pe_mid_server_create(const pe_mid_server_config* config, pe_mid_server** server) {
+  if (!server) return fail(PE_ERR_INVALID, "null argument");
+  MidServer* S = nullptr;
+  const pe_status st = mid_server_create(config, &S);
+  *server = reinterpret_cast<pe_mid_server*>(S);
+  return st;
+}
+
+pe_status pe_mid_server_submit(pe_mid_server* server, const pe_high_board* board, int64_t* ticket) {
+  if (!server || !board || !ticket) return fail(PE_ERR_INVALID, "null argument");
+  return mid_server_submit(reinterpret_cast<MidServer*>(server), *board, ticket);
+}
+
+pe_status pe_mid_server_poll(pe_mid_server* server, int64_t ticket, pe_halfsupers results[19], int* done,
+                             double* latency_ms) {
+  if (!server || !done) return fail(PE_ERR_INVALID, "null argument");
+  return mid_server_poll(reinterpret_cast<MidServer*>(server), ticket, results, done, latency_ms);
+}
+
+pe_status pe_mid_server_wait(pe_mid_server* server, int64_t ticket, pe_halfsupers results[19], double* latency_ms) {
+  if (!server) return fail(PE_ERR_INVALID, "null argument");
+  return mid_server_wait(reinterpret_cast<MidServer*>(server), ticket, results, latency_ms);
+}
+
+pe_status pe_mid_server_stats(pe_mid_server* server, pe_mid_server_info* stats) {
+  if (!server || !stats) return fail(PE_ERR_INVALID, "null argument");
+  mid_server_stats(reinterpret_cast<MidServer*>(server), stats);
+  return PE_OK;
+}
+
+pe_status pe_mid_server_destroy(pe_mid_server* server) {
+  if (server) mid_server_destroy(reinterpret_cast<MidServer*>(server));
   return PE_OK;
 }
 

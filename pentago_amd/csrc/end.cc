@@ -87,6 +87,52 @@ void section_field_tables(const Section& s, std::vector<uint64_t>* out, int32_t 
   }
 }
 
+pe_status section_move_tables(const Section& s, std::vector<uint16_t>* out, int32_t move_off[4], int32_t nmoves[4],
+                              std::string* err) {
+  const auto& T = tables();
+  const int turn = s.sum() & 1;
+  int shape[4];
+  if (!s.shape(shape)) { *err = "invalid section"; return PE_ERR_INVALID; }
+  out->clear();
+  for (int d = 0; d < 4; d++) {
+    move_off[d] = -1;
+    nmoves[d] = 9 - s.c[d][0] - s.c[d][1];
+    if (nmoves[d] <= 0) { nmoves[d] = 0; continue; }
+    // the child side of the moves depends only on (section, d): use the line at block 0
+    pe_line L;
+    std::memset(&L, 0, sizeof L);
+    std::memcpy(L.section.counts, s.c, sizeof s.c);
+    L.dimension = uint8_t(d);
+    L.length = uint8_t((shape[d] + 7) / 8);
+    LineHost h;
+    const pe_status st = make_line(L, &h, err);
+    if (st != PE_OK) return st;
+    const LineDev& D = h.dev;
+    const int cd = D.child_dim, g = D.inv_global, lm = D.reflection_moves[cd];
+    const uint8_t* cd_syms = h.refl_syms.data() + D.rs_off[cd];
+    move_off[d] = int32_t(out->size());
+    const int b = rmin_bucket(s.c[d][0], s.c[d][1]);
+    for (int i = T.rmin_offsets[b]; i < T.rmin_offsets[b + 1]; i++) {
+      const uint16_t quad = T.rmin_flat[i];
+      const int filled = T.unpack[quad][0] | T.unpack[quad][1];
+      uint16_t row[16] = {};
+      int m = 0;
+      for (int e = 0; e < 9; e++) {
+        if (filled >> e & 1) continue;
+        const int ir = T.rmin_inverse[quad + (1 + turn) * T.pack[1 << e]];
+        const int j = ir >> 2;
+        const int k = ((T.commute[g][(ir & 3) << (2 * d)] + (D.reflected ? cd_syms[j] : 0)) >> (2 * cd)) & 3;
+        const int kk = j >> 3, within = (j ^ int(j < lm)) & 7;
+        if (kk >= 64) { *err = "internal: child line longer than 64 blocks"; return PE_ERR_INVALID; }
+        row[m++] = uint16_t(kk | within << 6 | k << 9);
+      }
+      if (m != nmoves[d]) { *err = "internal: move count mismatch"; return PE_ERR_INVALID; }
+      out->insert(out->end(), row, row + 16);
+    }
+  }
+  return PE_OK;
+}
+
 pe_status make_line(const pe_line& line, LineHost* out, std::string* err) {
   const auto& T = tables();
   std::memset(&out->dev, 0, sizeof out->dev);

@@ -57,6 +57,7 @@ struct LineDev {
   int32_t ax_stride[4], ax_stride_last[4], ax_fcn[4], ax_moves[4], ax_rs[4];
   int32_t input_stride;                // stride of the child dimension inside a child block
   const int64_t* in_off;               // node offset of input block k (null: k*(block_stride+1))
+  int64_t in_base, in_kstride;         // fused plans: in_off[k] = in_base + k * in_kstride
   const int64_t* out_off;              // node offset of output block k (null: k*block_stride)
 };
 
@@ -88,6 +89,8 @@ struct SecDev {
   int32_t turn;
   int32_t rmin_off[4], rmin_size[4];
   int32_t field_off[4];     // start (in supers) of quadrant q's field table: 15 supers per rmin index
+  int32_t move_off[4];      // start (in u16 entries) of dimension d's move table rows
+  int32_t nmoves[4];        // moves per node along dimension d (0: closed)
 };
 // One output block: its section, coordinates, the line through it per dimension, output offset.
 struct BlockDev {
@@ -100,7 +103,7 @@ struct BlockDev {
 struct FusedPlan {
   void* blob = nullptr;
   int nblocks = 0;
-  size_t off_lines = 0, off_secs = 0, off_fields = 0, off_refl = 0;
+  size_t off_lines = 0, off_secs = 0, off_fields = 0, off_moves = 0, off_refl = 0;
   uint64_t* output = nullptr;
   int64_t output_nodes = 0;
 };
@@ -109,5 +112,14 @@ struct FusedPlan {
 // 60 words: for each super word k, tables [F0 (mover's side), F1 (other side), Fm (OR over
 // the mover's moves)] x 5 fields (word k of each super).
 void section_field_tables(const Section& s, std::vector<uint64_t>* out, int32_t field_off[4]);
+
+// Move tables of one section (fused kernel): for each open dimension d (move_off[d] >= 0,
+// nmoves[d] = empty cells of quadrant d) and rotation-minimal index i of quadrant d, a row
+// of 16 u16 entries, one per move in ascending cell order: child block along the child
+// dimension (bits 0-5), node index inside that block along it after the reflection swap
+// (bits 6-8) and the move's local rotation restricted to the child dimension (bits 9-10).
+// Everything here depends only on (section, d, i, move), not on the line's other blocks.
+pe_status section_move_tables(const Section& s, std::vector<uint16_t>* out, int32_t move_off[4], int32_t nmoves[4],
+                              std::string* err);
 
 }  // namespace pe

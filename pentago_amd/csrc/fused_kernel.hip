@@ -94,7 +94,8 @@ constexpr int kParts = 4096 / kThreads;
 __global__ __launch_bounds__(kThreads)
 void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __restrict__ lines,
                          const SecDev* __restrict__ secs, const uint64_t* __restrict__ fields,
-                         const uint8_t* __restrict__ refl, uint64_t* __restrict__ output, DevTables T) {
+                         const uint16_t* __restrict__ movetab, const uint8_t* __restrict__ refl,
+                         uint64_t* __restrict__ output) {
   const BlockDev& B = blocks[blockIdx.x / kParts];
   const SecDev& S = secs[B.sec];
   const int p = (blockIdx.x % kParts) * kThreads + threadIdx.x;
@@ -109,10 +110,10 @@ void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __r
     loc[1] = rem % bs1;
     loc[0] = rem / bs1;
   }
-  const int turn = S.turn;
 
-  // ---- children: per open dimension, OR the moves' children shifted along the child
-  //      dimension, then apply the rest of the local rotation and the rearrangement ----
+  // ---- children: per open dimension, AND the moves' children shifted along the child
+  //      dimension (OR of complements = complement of AND), then apply the rest of the
+  //      local rotation and the rearrangement ----
   S4 t0 = s4_zero(), t1 = s4_zero();
 #pragma unroll
   for (int d = 0; d < 4; d++) {
@@ -129,30 +130,25 @@ void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __r
       base_last += L.ax_stride_last[a] * x;
       if (L.reflected) local_sym |= syms[L.ax_rs[a] + loc[a]];
     }
-    const int g = L.inv_global;
-    const int gi = 8 * B.b[d] + loc[d];
-    const uint32_t qs = T.rmin_sides[S.rmin_off[d] + gi];
-    const uint32_t sq0 = turn ? qs >> 16 : qs & 0x1ff, sq1 = turn ? qs & 0x1ff : qs >> 16;
-    const int quad = T.rmin_flat[S.rmin_off[d] + gi];
-    const int line_moves = L.reflection_moves[cd];
-    const uint8_t* cd_syms = syms + L.rs_off[cd];
-    const uint64_t* in = L.input;
-    S4 acc0 = s4_zero(), acc1 = s4_zero();
-    for (uint32_t m = 511 & ~(sq0 | sq1); m; m &= m - 1) {
-      const uint32_t mv = m & (0u - m);
-      const int ir = T.rmin_inverse[quad + (1 + turn) * T.pack[mv]];
-      const int j = ir >> 2;
-      // local rotation of this move restricted to the child dimension
-      const int k = ((T.commute[g * 256 + ((ir & 3) << (2 * d))] + cd_syms[j]) >> (2 * cd)) & 3;
-      const int64_t cidx = L.in_off[j >> 3] + int64_t((j >> 3) == L.child_length - 1 ? base_last : base) +
-                           int64_t(L.input_stride) * ((j ^ int(j < line_moves)) & 7);
+    const uint16_t* row = movetab + S.move_off[d] + 16 * (8 * B.b[d] + loc[d]);
+    const int nm = S.nmoves[d];
+    const int last = L.child_length - 1;
+    const uint64_t* in = L.input + 8 * L.in_base;
+    S4 acc0, acc1;
+    for (int i = 0; i < 4; i++) acc0.w[i] = acc1.w[i] = ~uint64_t(0);
+    for (int m = 0; m < nm; m++) {
+      const int e = row[m];
+      const int kk = e & 63;
+      const int64_t cidx = int64_t(kk) * L.in_kstride + (kk == last ? base_last : base) +
+                           int64_t(L.input_stride) * ((e >> 6) & 7);
       const uint64_t* c = in + 8 * cidx;
-      acc0 = s4_or(acc0, shift_coord_var(s4_not(ld(c)), cd, k));
-      acc1 = s4_or(acc1, shift_coord_var(s4_not(ld(c + 4)), cd, k));
+      const int k = e >> 9;
+      acc0 = s4_and(acc0, shift_coord_var(ld(c), cd, k));
+      acc1 = s4_and(acc1, shift_coord_var(ld(c + 4), cd, k));
     }
     const int y = local_add(L.inv_local, local_sym);
-    t0 = s4_or(t0, rearrange(shift_local(acc0, y), g));
-    t1 = s4_or(t1, rearrange(shift_local(acc1, y), g));
+    t0 = s4_or(t0, rearrange(shift_local(s4_not(acc0), y), L.inv_global));
+    t1 = s4_or(t1, rearrange(shift_local(s4_not(acc1), y), L.inv_global));
   }
   const S4 r0 = rmax(t0), r1 = rmax(t1);
 
@@ -276,6 +272,9 @@ pe_status fused_plan_create(const pe_slab& out, int nch, const pe_slab* ch, Fuse
   sec.turn = s.sum() & 1;
   std::vector<uint64_t> fields;
   section_field_tables(s, &fields, sec.field_off);
+  std::vector<uint16_t> movetab;
+  st = section_move_tables(s, &movetab, sec.move_off, sec.nmoves, err);
+  if (st != PE_OK) return st;
   const auto& Tb = tables();
   for (int a = 0; a < 4; a++) {
     const int b = rmin_bucket(s.c[a][0], s.c[a][1]);
@@ -311,6 +310,15 @@ pe_status fused_plan_create(const pe_slab& out, int nch, const pe_slab* ch, Fuse
   std::vector<int64_t> at(lines.size());
   for (size_t k = 0; k < lines.size(); k++) {
     devs[k] = lines[k].dev;
+    // child blocks along the child dimension sit at affine offsets in their slab
+    const std::vector<int64_t>& io = lines[k].in_off;
+    devs[k].in_base = io[0];
+    devs[k].in_kstride = io.size() > 1 ? io[1] - io[0] : 0;
+    for (size_t kk = 0; kk < io.size(); kk++)
+      if (io[kk] != devs[k].in_base + int64_t(kk) * devs[k].in_kstride) {
+        *err = "internal: child blocks of a line are not evenly spaced";
+        return PE_ERR_INVALID;
+      }
     devs[k].refl_base = int32_t(refl.size());
     refl.insert(refl.end(), lines[k].refl_syms.begin(), lines[k].refl_syms.end());
     at[k] = int64_t(offs.size());
@@ -321,7 +329,8 @@ pe_status fused_plan_create(const pe_slab& out, int nch, const pe_slab* ch, Fuse
   const size_t o_lines = up(sizeof(BlockDev) * blocks.size());
   const size_t o_secs = o_lines + up(sizeof(LineDev) * devs.size());
   const size_t o_fields = o_secs + up(sizeof(SecDev));
-  const size_t o_offs = o_fields + up(sizeof(uint64_t) * fields.size());
+  const size_t o_moves = o_fields + up(sizeof(uint64_t) * fields.size());
+  const size_t o_offs = o_moves + up(sizeof(uint16_t) * std::max<size_t>(movetab.size(), 1));
   const size_t o_refl = o_offs + up(sizeof(int64_t) * offs.size());
   const size_t bytes = o_refl + refl.size();
   void* d = nullptr;
@@ -333,6 +342,7 @@ pe_status fused_plan_create(const pe_slab& out, int nch, const pe_slab* ch, Fuse
   std::memcpy(blob.data() + o_lines, devs.data(), sizeof(LineDev) * devs.size());
   std::memcpy(blob.data() + o_secs, &sec, sizeof sec);
   std::memcpy(blob.data() + o_fields, fields.data(), sizeof(uint64_t) * fields.size());
+  if (!movetab.empty()) std::memcpy(blob.data() + o_moves, movetab.data(), sizeof(uint16_t) * movetab.size());
   std::memcpy(blob.data() + o_offs, offs.data(), sizeof(int64_t) * offs.size());
   std::memcpy(blob.data() + o_refl, refl.data(), refl.size());
   if (hipMemcpy(d, blob.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
@@ -345,6 +355,7 @@ pe_status fused_plan_create(const pe_slab& out, int nch, const pe_slab* ch, Fuse
   plan->off_lines = o_lines;
   plan->off_secs = o_secs;
   plan->off_fields = o_fields;
+  plan->off_moves = o_moves;
   plan->off_refl = o_refl;
   plan->output = static_cast<uint64_t*>(out.data);
   plan->output_nodes = nodes;
@@ -362,7 +373,8 @@ pe_status fused_plan_run(const FusedPlan& plan, hipStream_t stream) {
   hipLaunchKernelGGL(fused_blocks_kernel, dim3(unsigned(grid)), dim3(kThreads), 0, stream,
                      reinterpret_cast<const BlockDev*>(base), reinterpret_cast<const LineDev*>(base + plan.off_lines),
                      reinterpret_cast<const SecDev*>(base + plan.off_secs),
-                     reinterpret_cast<const uint64_t*>(base + plan.off_fields), base + plan.off_refl, plan.output, T);
+                     reinterpret_cast<const uint64_t*>(base + plan.off_fields),
+                     reinterpret_cast<const uint16_t*>(base + plan.off_moves), base + plan.off_refl, plan.output);
   PE_HIP(hipGetLastError(), "fused_blocks_kernel launch");
   return PE_OK;
 }

@@ -130,8 +130,9 @@ struct Event {
 }  // namespace
 
 const char* mid_plan_build_schedule(const std::vector<MidInfo>& infos, const MidConfig& cfg,
-                                    std::vector<MidTask>* order, std::vector<int2>* deps,
-                                    std::vector<int64_t>* offsets, int64_t* arena_bytes, int32_t* nslots) {
+                                    const std::vector<int64_t>* fixed_offsets, std::vector<MidTask>* order,
+                                    std::vector<int2>* deps, std::vector<int64_t>* offsets, int64_t* arena_bytes,
+                                    int32_t* nslots) {
   const int nb = int(infos.size());
   std::vector<BoardSched> B(nb);
   int32_t slot_count = 0;
@@ -139,7 +140,7 @@ const char* mid_plan_build_schedule(const std::vector<MidInfo>& infos, const Mid
     const MidInfo& I = infos[b];
     const MidSizes s = mid_sizes(I);
     B[b].bytes = int64_t(mid_board_bytes(I));
-    if (B[b].bytes > cfg.budget) return "a board needs more workspace than the budget";
+    if (!fixed_offsets && B[b].bytes > cfg.budget) return "a board needs more workspace than the budget";
     B[b].steps.resize(I.spots + 3);
     B[b].steps[0].entries = 0;
     B[b].steps[1].entries = s.set0 + s.wins1 + s.cs1ps;
@@ -177,6 +178,19 @@ const char* mid_plan_build_schedule(const std::vector<MidInfo>& infos, const Mid
     for (int c = 0; c < B[b].steps[step].tasks; c++) ready.push(Ready{B[b].admit_seq, step, c, b});
   };
   auto try_admit = [&]() {
+    if (fixed_offsets) {   // every board has its own region already: all admitted at once
+      for (auto& kv : pending)
+        for (int b : kv.second) {
+          slots.push_back(Slot{(*fixed_offsets)[b], B[b].bytes, b});
+          carved = std::max(carved, (*fixed_offsets)[b] + B[b].bytes);
+          B[b].offset = (*fixed_offsets)[b];
+          board_slot[b] = int(slots.size()) - 1;
+          B[b].admit_seq = admit_seq++;
+          make_ready(b, 0);
+        }
+      pending.clear();
+      return;
+    }
     for (auto it = pending.begin(); it != pending.end();) {
       const int64_t bytes = it->first;
       auto& q = it->second;
@@ -308,59 +322,80 @@ const char* mid_plan_build_schedule(const std::vector<MidInfo>& infos, const Mid
         if (B[b].offset != slots[k].offset || ++chain > nb) return "internal: slot chain broken";
       }
     }
-    if (carved > cfg.budget) return "internal: workspace over budget";
+    if (!fixed_offsets && carved > cfg.budget) return "internal: workspace over budget";
   }
   return nullptr;
+}
+
+pe_status mid_plan_prepare(const std::vector<MidInfo>& infos, const MidConfig& cfg,
+                           const std::vector<int64_t>* fixed_offsets, MidPrepared* p) {
+  if (const char* e = mid_plan_build_schedule(infos, cfg, fixed_offsets, &p->order, &p->deps, &p->offsets,
+                                              &p->arena_bytes, &p->nslots)) {
+    set_error(e);
+    return std::strncmp(e, "internal", 8) ? PE_ERR_CAPACITY : PE_ERR_INVALID;
+  }
+  auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+  const size_t nb = infos.size();
+  p->o_tasks = up(sizeof(MidBoardDev) * nb);
+  p->o_deps = p->o_tasks + up(sizeof(MidTask) * p->order.size());
+  p->o_res = p->o_deps + up(sizeof(int2) * std::max<size_t>(p->deps.size(), 1));
+  p->o_state = p->o_res + up(sizeof(pe_halfsupers) * 19 * std::max<size_t>(nb, 1));
+  p->o_head = p->o_state + up(sizeof(int32_t) * size_t(p->nslots));
+  p->blob_bytes = p->o_head + 256;
+  return PE_OK;
+}
+
+void mid_plan_fill(const std::vector<MidInfo>& infos, const MidConfig& cfg, const MidPrepared& p, void* dev_blob,
+                   uint8_t* arena, const DevTables& T, std::vector<uint8_t>* host_blob, MidPlan* plan) {
+  const int nb = int(infos.size());
+  auto* base = static_cast<uint8_t*>(dev_blob);
+  std::vector<MidBoardDev> boards(nb);
+  for (int b = 0; b < nb; b++) {
+    boards[b].I = infos[b];
+    mid_carve(&boards[b], arena + p.offsets[b]);
+    boards[b].results = reinterpret_cast<pe_halfsupers*>(base + p.o_res) + 19 * size_t(b);
+  }
+  host_blob->assign(p.o_state, 0);
+  std::memcpy(host_blob->data(), boards.data(), sizeof(MidBoardDev) * nb);
+  std::memcpy(host_blob->data() + p.o_tasks, p.order.data(), sizeof(MidTask) * p.order.size());
+  if (!p.deps.empty()) std::memcpy(host_blob->data() + p.o_deps, p.deps.data(), sizeof(int2) * p.deps.size());
+  plan->nb = nb;
+  plan->ntasks = int64_t(p.order.size());
+  plan->nslots = p.nslots;
+  plan->cfg = cfg;
+  plan->spots.resize(nb);
+  for (int b = 0; b < nb; b++) plan->spots[b] = infos[b].spots;
+  plan->arena_bytes = p.arena_bytes;
+  plan->boards = reinterpret_cast<const MidBoardDev*>(base);
+  plan->tasks = reinterpret_cast<const MidTask*>(base + p.o_tasks);
+  plan->deps = reinterpret_cast<const int2*>(base + p.o_deps);
+  plan->results = reinterpret_cast<pe_halfsupers*>(base + p.o_res);
+  plan->counters = reinterpret_cast<int32_t*>(base + p.o_state);
+  plan->head = reinterpret_cast<unsigned long long*>(base + p.o_head);
+  plan->error = reinterpret_cast<int32_t*>(base + p.o_head + 64);
+  plan->tables = T;
 }
 
 pe_status mid_plan_create(const std::vector<MidInfo>& infos, const MidConfig& cfg, MidPlan* plan) {
   DevTables T;
   std::string err;
   if (device_tables(&T, &err) != PE_OK) { set_error(err); return PE_ERR_DEVICE; }
-  const int nb = int(infos.size());
-  std::vector<MidTask> order;
-  std::vector<int2> deps;
-  std::vector<int64_t> offsets;
-  int64_t arena_bytes = 0;
-  int32_t nslots = 0;
-  if (const char* e = mid_plan_build_schedule(infos, cfg, &order, &deps, &offsets, &arena_bytes, &nslots)) {
-    set_error(e);
-    return std::strncmp(e, "internal", 8) ? PE_ERR_CAPACITY : PE_ERR_INVALID;
-  }
+  MidPrepared p;
+  pe_status st = mid_plan_prepare(infos, cfg, nullptr, &p);
+  if (st != PE_OK) return st;
   plan->device = 0;
   PE_HIP(hipGetDevice(&plan->device), "hipGetDevice");
-  plan->nb = nb;
-  plan->ntasks = int64_t(order.size());
-  plan->nslots = nslots;
-  plan->cfg = cfg;
-  plan->spots.resize(nb);
-  for (int b = 0; b < nb; b++) plan->spots[b] = infos[b].spots;
-  auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
-  const size_t o_tasks = up(sizeof(MidBoardDev) * size_t(nb));
-  const size_t o_deps = o_tasks + up(sizeof(MidTask) * order.size());
-  const size_t o_res = o_deps + up(sizeof(int2) * std::max<size_t>(deps.size(), 1));
-  const size_t o_state = o_res + up(sizeof(pe_halfsupers) * 19 * size_t(std::max(nb, 1)));
-  const size_t bytes = o_state + up(sizeof(int32_t) * size_t(nslots)) + 256;
   void* d = nullptr;
-  PE_HIP(hipMalloc(&d, bytes), "hipMalloc(mid plan)");
+  PE_HIP(hipMalloc(&d, p.blob_bytes), "hipMalloc(mid plan)");
   void* arena = nullptr;
-  if (hipMalloc(&arena, size_t(std::max<int64_t>(arena_bytes, 256))) != hipSuccess) {
+  if (hipMalloc(&arena, size_t(std::max<int64_t>(p.arena_bytes, 256))) != hipSuccess) {
     (void)hipFree(d);
     set_error("hipMalloc(mid workspace arena) failed");
     return PE_ERR_DEVICE;
   }
-  auto* base = static_cast<uint8_t*>(d);
-  std::vector<MidBoardDev> boards(nb);
-  for (int b = 0; b < nb; b++) {
-    boards[b].I = infos[b];
-    mid_carve(&boards[b], static_cast<uint8_t*>(arena) + offsets[b]);
-    boards[b].results = reinterpret_cast<pe_halfsupers*>(base + o_res) + 19 * size_t(b);
-  }
-  std::vector<uint8_t> blob(o_state, 0);
-  std::memcpy(blob.data(), boards.data(), sizeof(MidBoardDev) * nb);
-  std::memcpy(blob.data() + o_tasks, order.data(), sizeof(MidTask) * order.size());
-  if (!deps.empty()) std::memcpy(blob.data() + o_deps, deps.data(), sizeof(int2) * deps.size());
-  if (hipMemcpy(d, blob.data(), o_state, hipMemcpyHostToDevice) != hipSuccess) {
+  std::vector<uint8_t> blob;
+  mid_plan_fill(infos, cfg, p, d, static_cast<uint8_t*>(arena), T, &blob, plan);
+  if (hipMemcpy(d, blob.data(), blob.size(), hipMemcpyHostToDevice) != hipSuccess) {
     (void)hipFree(d);
     (void)hipFree(arena);
     set_error("hipMemcpy(mid plan) failed");
@@ -368,15 +403,6 @@ pe_status mid_plan_create(const std::vector<MidInfo>& infos, const MidConfig& cf
   }
   plan->blob = d;
   plan->arena = arena;
-  plan->arena_bytes = arena_bytes;
-  plan->boards = reinterpret_cast<const MidBoardDev*>(base);
-  plan->tasks = reinterpret_cast<const MidTask*>(base + o_tasks);
-  plan->deps = reinterpret_cast<const int2*>(base + o_deps);
-  plan->results = reinterpret_cast<pe_halfsupers*>(base + o_res);
-  plan->counters = reinterpret_cast<int32_t*>(base + o_state);
-  plan->head = reinterpret_cast<unsigned long long*>(base + o_state + up(sizeof(int32_t) * size_t(nslots)));
-  plan->error = reinterpret_cast<int32_t*>(reinterpret_cast<uint8_t*>(plan->head) + 64);
-  plan->tables = T;
   return PE_OK;
 }
 
