@@ -159,6 +159,29 @@ def mid_bench(P, hip, n_boards=4096, seed=2):
     for sp in spots:
         sp = int(sp)
         work += 2 * 16 * sum(comb(sp, n // 2) * comb(sp - n // 2, n - n // 2) * (sp - n + 1) for n in range(sp + 1))
+    # small boards alone (24..30 stones, 6..12 empty cells): launch overhead dominates the
+    # plain path, the batched kernel solves them one workgroup per board
+    small = seeded_boards(n_boards, seed + 1, 24, 30)
+    plan = P.MidPlan(small)
+    plan.run(stream.ptr)
+    sres = plan.results(stream.ptr)
+    ev0.record(stream)
+    plan.run(stream.ptr)
+    ev1.record(stream)
+    ev1.synchronize()
+    small_ms = ev0.elapsed_ms(ev1)
+    plan.close()
+    splain = np.zeros_like(sres)
+    t0 = time.perf_counter()
+    for i, b in enumerate(small):
+        P.midsolve_internal_raw(b, splain[i])
+    small_plain_s = time.perf_counter() - t0
+    out["small_boards"] = {"workload": "%d seeded boards, 24..30 stones (6..12 empty cells)" % n_boards,
+                           "batched_boards_per_s": n_boards / (small_ms / 1e3), "batched_ms": small_ms,
+                           "plain_boards_per_s": n_boards / small_plain_s,
+                           "batched_over_plain": small_plain_s / (small_ms / 1e3),
+                           "mismatches": int(np.sum(np.any(splain != sres, axis=(1, 2))))}
+    log("midsolve small boards: %s" % out["small_boards"])
     out.update({
         "config": {"workload": "%d seeded boards, 18..30 stones (6..18 empty cells)" % n_boards,
                    "spots_histogram": {int(k): int(v) for k, v in zip(*np.unique(spots, return_counts=True))}},

@@ -179,53 +179,60 @@ pe_status pe_midsolve(const pe_high_board* board, pe_mid_value* values, int* cou
 pe_status pe_midsolve_batch(int n, const pe_high_board* boards, pe_halfsupers (*results)[19], int mode,
                             void* stream);
 
-/* Work-queue plans.  Every board of a batch (any mix of 6..18 empty cells) runs in one
- * persistent launch whose workgroups pop tasks from a device queue: per board a slot gate,
- * the table pass and one task group per level, each level waiting only for the same
- * board's previous level.  Workspace is held inside an HBM budget: a board is admitted when
- * a slot frees up.  No counterpart in the reference, which solves one board per call
+/* Batched plans.  A batch of boards of any mix of sizes runs as: small boards (at most
+ * small_spots empty cells) in one persistent kernel whose workgroups pop boards from a device
+ * queue and solve each whole board (every level, workgroup barriers between levels) in a
+ * workspace slot of their own; large boards level by level, one full-device launch per
+ * level, on `streams` streams of their own that overlap with the small-board kernel.
+ * Memory is held within workspace_budget (fewer workgroups / streams when it is tight).
+ * No counterpart in the reference, which solves one board per call
  * (pentago/mid/midengine.cc:57-125); each board's results equal pe_midsolve_internal's.
  * Zero fields take the device defaults (set by pe_mid_autotune, else built in). */
 typedef struct {
-  int64_t workspace_budget;   /* bytes of workspace held at once (0: 60% of free HBM, <= 200 GB) */
-  int32_t tile;               /* minimum entries per task */
-  int32_t workgroups;         /* persistent workgroups of 256 threads */
-  int64_t wait_limit_us;      /* bound on one dependency wait; exceeded -> PE_ERR_DEVICE (0: 10 s) */
+  int64_t workspace_budget;   /* bytes held by one run (0: 60% of free HBM, <= 200 GB) */
+  int32_t small_spots;        /* boards with at most this many empty cells: one workgroup each (0: 14) */
+  int32_t workgroups;         /* persistent workgroups for small boards (0: a full device) */
+  int32_t threads;            /* threads per small-board workgroup: 256, 512, 1024 (0: 512) */
+  int32_t streams;            /* streams for large boards (0: 2; negative: all on the run stream) */
   int32_t use_graph;          /* nonzero: pe_mid_plan_run replays a captured hipGraph */
   int32_t pad;
 } pe_mid_config;
+typedef struct {
+  int64_t small_boards, large_boards, launches, workspace_bytes;
+  int32_t workgroups, streams;
+} pe_mid_plan_stats;
 typedef struct pe_mid_plan pe_mid_plan;
 pe_status pe_mid_plan_create(int n, const pe_high_board* boards, const pe_mid_config* config, pe_mid_plan** plan);
 /* Asynchronous on `stream`; a plan may be run any number of times (not concurrently). */
 pe_status pe_mid_plan_run(pe_mid_plan* plan, void* stream);
 /* Waits for the last run on `stream` and copies results[i][0..18] (entries past spots zero). */
 pe_status pe_mid_plan_results(pe_mid_plan* plan, pe_halfsupers (*results)[19], void* stream);
-/* Tasks in the queue, workspace bytes held, and the configuration in effect. */
-pe_status pe_mid_plan_info(const pe_mid_plan* plan, int64_t* tasks, int64_t* workspace_bytes, pe_mid_config* config);
+/* How the batch was split and the configuration in effect. */
+pe_status pe_mid_plan_info(const pe_mid_plan* plan, pe_mid_plan_stats* stats, pe_mid_config* config);
 pe_status pe_mid_plan_destroy(pe_mid_plan* plan);
-/* Host-only: build and check the task order and workspace slots a plan would use (no
- * device calls; every config field must be set).  Returns the task count and workspace. */
-pe_status pe_mid_schedule_info(int n, const pe_high_board* boards, const pe_mid_config* config, int64_t* tasks,
+/* Host-only: the launches per run and the device memory a plan with this (complete)
+ * configuration would use, or PE_ERR_CAPACITY.  No device calls. */
+pe_status pe_mid_schedule_info(int n, const pe_high_board* boards, const pe_mid_config* config, int64_t* launches,
                                int64_t* workspace_bytes);
-/* Time candidate (tile, workgroups) pairs on these boards and keep the fastest as the
- * current device's default.  The chosen configuration is returned through *best. */
+/* Time (small_spots, threads) pairs on these boards and keep the fastest as the current
+ * device's default.  The chosen configuration is returned through *best. */
 pe_status pe_mid_autotune(int n, const pe_high_board* boards, int64_t workspace_budget, pe_mid_config* best);
 
 /* Serving loop: single-position solve requests admitted as they arrive.  A worker thread
- * batches the pending requests whose workspace fits the free part of an HBM pool and
- * launches each batch on one of several streams through the work-queue kernel, so requests
- * that arrive while others are on the device start as soon as memory and a stream free up;
- * batches overlap on the device.  Results are those of pe_midsolve_internal.  The
+ * batches the pending requests whose device memory fits the part of the HBM budget not held
+ * by batches in flight and runs each batch on one of several streams as a batched plan, so
+ * requests that arrive while others are on the device start as soon as memory and a stream
+ * free up; batches overlap on the device.  Results are those of pe_midsolve_internal.  The
  * reference serves one wasm midsolve per request (web/server/index.js:16-58); this has no
  * direct counterpart.  All calls are thread-safe; each ticket is taken by exactly one
  * successful poll (done = 1) or wait. */
 typedef struct pe_mid_server pe_mid_server;
 typedef struct {
-  int64_t workspace_budget;   /* bytes of the workspace pool (0: 40% of free HBM, <= 100 GB) */
+  int64_t workspace_budget;   /* bytes all batches in flight may hold (0: 40% of free HBM, <= 100 GB) */
   int32_t streams;            /* batches in flight at once (0: 4) */
   int32_t max_batch;          /* requests per batch (0: 512) */
-  int32_t tile;               /* work-queue tile (0: 1024) */
-  int32_t workgroups;         /* work-queue workgroups per batch (0: one full device) */
+  int32_t small_spots;        /* batched-plan small-board limit (0: 14) */
+  int32_t workgroups;         /* small-board workgroups per batch (0: a device's worth / streams) */
 } pe_mid_server_config;
 typedef struct {
   int64_t submitted, completed, batches, peak_workspace, pending, in_flight;

@@ -91,18 +91,23 @@ class HalfSupers(C.Structure):
 
 class MidConfig(C.Structure):
     """pe_mid_config: zero fields take the device defaults."""
-    _fields_ = [("workspace_budget", C.c_int64), ("tile", C.c_int32), ("workgroups", C.c_int32),
-                ("wait_limit_us", C.c_int64), ("use_graph", C.c_int32), ("pad", C.c_int32)]
+    _fields_ = [("workspace_budget", C.c_int64), ("small_spots", C.c_int32), ("workgroups", C.c_int32),
+                ("threads", C.c_int32), ("streams", C.c_int32), ("use_graph", C.c_int32), ("pad", C.c_int32)]
 
     def as_dict(self):
-        return {"workspace_budget": int(self.workspace_budget), "tile": int(self.tile),
-                "workgroups": int(self.workgroups), "wait_limit_us": int(self.wait_limit_us),
+        return {"workspace_budget": int(self.workspace_budget), "small_spots": int(self.small_spots),
+                "workgroups": int(self.workgroups), "threads": int(self.threads), "streams": int(self.streams),
                 "use_graph": bool(self.use_graph)}
+
+
+class MidPlanStats(C.Structure):
+    _fields_ = [("small_boards", C.c_int64), ("large_boards", C.c_int64), ("launches", C.c_int64),
+                ("workspace_bytes", C.c_int64), ("workgroups", C.c_int32), ("streams", C.c_int32)]
 
 
 class MidServerConfig(C.Structure):
     _fields_ = [("workspace_budget", C.c_int64), ("streams", C.c_int32), ("max_batch", C.c_int32),
-                ("tile", C.c_int32), ("workgroups", C.c_int32)]
+                ("small_spots", C.c_int32), ("workgroups", C.c_int32)]
 
 
 class MidServerInfo(C.Structure):
@@ -160,7 +165,7 @@ def lib():
             "pe_mid_plan_create": (C.c_int, [C.c_int, P(HighBoard), P(MidConfig), P(V)]),
             "pe_mid_plan_run": (C.c_int, [V, V]),
             "pe_mid_plan_results": (C.c_int, [V, V, V]),
-            "pe_mid_plan_info": (C.c_int, [V, P(C.c_int64), P(C.c_int64), P(MidConfig)]),
+            "pe_mid_plan_info": (C.c_int, [V, P(MidPlanStats), P(MidConfig)]),
             "pe_mid_plan_destroy": (C.c_int, [V]),
             "pe_mid_schedule_info": (C.c_int, [C.c_int, P(HighBoard), P(MidConfig), P(C.c_int64), P(C.c_int64)]),
             "pe_mid_autotune": (C.c_int, [C.c_int, P(HighBoard), C.c_int64, P(MidConfig)]),
@@ -393,16 +398,18 @@ def results_to_lists(boards, res):
 
 
 class MidPlan:
-    """A batch of boards solved by the persistent work-queue kernel (pe_mid_plan_*).
+    """A batch of boards solved by a batched plan (pe_mid_plan_*): small boards one per
+    workgroup through a persistent device-queue kernel, large boards level by level.
 
     Build once, run any number of times; results() waits for the last run and returns an
     (n, 19, 4) uint64 array (win lo, win hi, notlose lo, notlose hi per entry)."""
 
-    def __init__(self, boards, workspace_budget=0, tile=0, workgroups=0, wait_limit_us=0, use_graph=False):
+    def __init__(self, boards, workspace_budget=0, small_spots=0, workgroups=0, threads=0, streams=0,
+                 use_graph=False):
         self.boards = list(boards)
         n = len(self.boards)
         arr = (HighBoard * max(n, 1))(*self.boards)
-        cfg = MidConfig(workspace_budget, tile, workgroups, wait_limit_us, int(bool(use_graph)), 0)
+        cfg = MidConfig(workspace_budget, small_spots, workgroups, threads, streams, int(bool(use_graph)), 0)
         self.ptr = C.c_void_p()
         _check(lib().pe_mid_plan_create(n, arr, C.byref(cfg), C.byref(self.ptr)), "pe_mid_plan_create")
 
@@ -416,10 +423,10 @@ class MidPlan:
         return res[:len(self.boards)]
 
     def info(self):
-        tasks, ws, cfg = C.c_int64(), C.c_int64(), MidConfig()
-        _check(lib().pe_mid_plan_info(self.ptr, C.byref(tasks), C.byref(ws), C.byref(cfg)), "pe_mid_plan_info")
+        st, cfg = MidPlanStats(), MidConfig()
+        _check(lib().pe_mid_plan_info(self.ptr, C.byref(st), C.byref(cfg)), "pe_mid_plan_info")
         d = cfg.as_dict()
-        d.update(tasks=int(tasks.value), workspace_bytes=int(ws.value))
+        d.update({n: int(getattr(st, n)) for n, _ in MidPlanStats._fields_})
         return d
 
     def close(self):
@@ -434,14 +441,14 @@ class MidPlan:
             pass
 
 
-def mid_schedule_info(boards, workspace_budget, tile, workgroups):
-    """pe_mid_schedule_info: (tasks, workspace bytes) of the schedule a plan would use."""
+def mid_schedule_info(boards, workspace_budget, small_spots=14, workgroups=1024, threads=512, streams=2):
+    """pe_mid_schedule_info: (launches per run, device bytes) of a plan, host only."""
     n = len(boards)
     arr = (HighBoard * max(n, 1))(*boards)
-    cfg = MidConfig(workspace_budget, tile, workgroups, 0, 0, 0)
-    tasks, ws = C.c_int64(), C.c_int64()
-    _check(lib().pe_mid_schedule_info(n, arr, C.byref(cfg), C.byref(tasks), C.byref(ws)), "pe_mid_schedule_info")
-    return int(tasks.value), int(ws.value)
+    cfg = MidConfig(workspace_budget, small_spots, workgroups, threads, streams, 0, 0)
+    launches, ws = C.c_int64(), C.c_int64()
+    _check(lib().pe_mid_schedule_info(n, arr, C.byref(cfg), C.byref(launches), C.byref(ws)), "pe_mid_schedule_info")
+    return int(launches.value), int(ws.value)
 
 
 def mid_autotune(boards, workspace_budget=0):
@@ -460,8 +467,8 @@ class MidServer:
     submit() returns a ticket; wait(ticket) / poll(ticket) return (results, latency_ms) with
     results the list of (win, notlose) halfsuper ints as midsolve_internal gives."""
 
-    def __init__(self, workspace_budget=0, streams=0, max_batch=0, tile=0, workgroups=0):
-        cfg = MidServerConfig(workspace_budget, streams, max_batch, tile, workgroups)
+    def __init__(self, workspace_budget=0, streams=0, max_batch=0, small_spots=0, workgroups=0):
+        cfg = MidServerConfig(workspace_budget, streams, max_batch, small_spots, workgroups)
         self.ptr = C.c_void_p()
         self.spots = {}
         _check(lib().pe_mid_server_create(C.byref(cfg), C.byref(self.ptr)), "pe_mid_server_create")

@@ -24,9 +24,9 @@ void fused_plan_destroy(FusedPlan* plan);
 pe_status slab_line_passes(const pe_slab& out, int nch, const pe_slab* ch, std::vector<EndPlan>* passes,
                            int64_t* nodes_out, std::string* err);
 pe_status launch_fill_seeded(void* d_nodes, int64_t nodes, uint64_t seed, int64_t first, hipStream_t stream);
-size_t mid_board_bytes(const MidInfo& I);
+
 pe_status mid_solve_levels(const std::vector<MidInfo>& infos, uint8_t* arena, std::vector<size_t>* result_offsets,
-                           hipStream_t stream);
+                           hipStream_t stream, DeviceBuffer* desc_cache);
 int mid_queue_blocks_per_cu();
 
 namespace {
@@ -323,7 +323,7 @@ int64_t pe_midsolve_workspace_size(int min_slice) {
 
 namespace {
 
-struct TunedMid { bool set = false; int32_t tile = 0, workgroups = 0; };
+struct TunedMid { bool set = false; int32_t small_spots = 0, threads = 0, workgroups = 0; };
 TunedMid g_tuned[kMaxDevices];
 
 pe_status mid_infos(int n, const pe_high_board* boards, std::vector<MidInfo>* infos) {
@@ -338,9 +338,8 @@ pe_status mid_infos(int n, const pe_high_board* boards, std::vector<MidInfo>* in
 pe_status mid_resolve_config(const pe_mid_config* in, MidConfig* out) {
   int dev = 0;
   PE_HIP(hipGetDevice(&dev), "hipGetDevice");
-  int cus = 0, rate_khz = 0;
+  int cus = 0;
   PE_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute(CUs)");
-  PE_HIP(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev), "hipDeviceGetAttribute(wall clock)");
   pe_mid_config c{};
   if (in) c = *in;
   if (c.workspace_budget <= 0) {
@@ -349,22 +348,33 @@ pe_status mid_resolve_config(const pe_mid_config* in, MidConfig* out) {
     c.workspace_budget = std::min<int64_t>(int64_t(200e9), int64_t(0.6 * double(free_b)));
   }
   const TunedMid& t = g_tuned[dev];
-  if (c.tile <= 0) c.tile = t.set ? t.tile : 4096;
-  if (c.workgroups <= 0) c.workgroups = t.set ? t.workgroups : cus * std::max(1, mid_queue_blocks_per_cu());
-  if (c.wait_limit_us <= 0) c.wait_limit_us = 10 * 1000 * 1000;
+  if (c.small_spots <= 0) c.small_spots = t.set ? t.small_spots : 14;
+  if (c.threads <= 0) c.threads = t.set ? t.threads : 512;
+  if (c.threads != 256 && c.threads != 512 && c.threads != 1024) return fail(PE_ERR_INVALID, "threads must be 256, 512 or 1024");
+  if (c.workgroups <= 0) c.workgroups = t.set ? t.workgroups : cus * mid_board_blocks_per_cu(c.threads);
+  if (c.streams < 0) c.streams = 0;
+  else if (c.streams == 0) c.streams = 2;
   out->budget = c.workspace_budget;
-  out->tile = c.tile;
+  out->small_spots = c.small_spots;
   out->workgroups = c.workgroups;
-  out->wait_limit = c.wait_limit_us * int64_t(std::max(rate_khz, 1)) / 1000;
+  out->threads = c.threads;
+  out->streams = c.streams;
   out->graph = c.use_graph != 0;
   return PE_OK;
+}
+
+pe_status mid_run_sync(MidPlan& plan, pe_halfsupers (*results)[19], hipStream_t s) {
+  pe_status st = mid_plan_run(plan, s);
+  if (st == PE_OK) st = mid_plan_copy_results(plan, results, s);
+  PE_HIP(hipStreamSynchronize(s), "midsolve plan");
+  if (st == PE_OK) mid_plan_zero_tail(plan, results);
+  return st;
 }
 
 }  // namespace
 
 struct pe_mid_plan {
   MidPlan plan;
-  int64_t wait_limit_us;
 };
 
 extern "C" {
@@ -379,36 +389,37 @@ pe_status pe_mid_plan_create(int n, const pe_high_board* boards, const pe_mid_co
   st = mid_resolve_config(config, &cfg);
   if (st != PE_OK) return st;
   auto* p = new pe_mid_plan;
-  p->wait_limit_us = config && config->wait_limit_us > 0 ? config->wait_limit_us : 10 * 1000 * 1000;
   st = mid_plan_create(infos, cfg, &p->plan);
-  if (st != PE_OK) { delete p; return st; }
+  if (st != PE_OK) {
+    mid_plan_destroy(&p->plan);
+    delete p;
+    return st;
+  }
   *plan = p;
   return PE_OK;
 }
 
-pe_status pe_mid_schedule_info(int n, const pe_high_board* boards, const pe_mid_config* config, int64_t* tasks,
+pe_status pe_mid_schedule_info(int n, const pe_high_board* boards, const pe_mid_config* config, int64_t* launches,
                                int64_t* workspace_bytes) {
-  if (n < 0 || (n && !boards) || !config || config->workspace_budget <= 0 || config->tile <= 0 ||
-      config->workgroups <= 0)
+  if (n < 0 || (n && !boards) || !config || config->workspace_budget <= 0 || config->small_spots <= 0 ||
+      config->workgroups <= 0 || config->threads <= 0)
     return fail(PE_ERR_INVALID, "schedule info needs a board list and a complete configuration");
   std::vector<MidInfo> infos;
   pe_status st = mid_infos(n, boards, &infos);
   if (st != PE_OK) return st;
   MidConfig cfg;
   cfg.budget = config->workspace_budget;
-  cfg.tile = config->tile;
+  cfg.small_spots = config->small_spots;
   cfg.workgroups = config->workgroups;
-  cfg.wait_limit = 0;
+  cfg.threads = config->threads;
+  cfg.streams = std::max(config->streams, 0);
   cfg.graph = false;
-  std::vector<MidTask> order;
-  std::vector<int2> deps;
-  std::vector<int64_t> offsets;
-  int64_t arena = 0;
-  int32_t nslots = 0;
-  if (const char* e = mid_plan_build_schedule(infos, cfg, nullptr, &order, &deps, &offsets, &arena, &nslots))
-    return fail(std::strncmp(e, "internal", 8) ? PE_ERR_CAPACITY : PE_ERR_INVALID, e);
-  if (tasks) *tasks = int64_t(order.size());
-  if (workspace_bytes) *workspace_bytes = arena;
+  MidLayout L;
+  if (const char* e = mid_layout(infos, cfg, &L)) return fail(PE_ERR_CAPACITY, e);
+  int64_t l = L.small.empty() ? 0 : 3;
+  for (int b : L.big) l += 2 + infos[size_t(b)].spots;
+  if (launches) *launches = l;
+  if (workspace_bytes) *workspace_bytes = int64_t(L.bytes);
   return PE_OK;
 }
 
@@ -419,20 +430,33 @@ pe_status pe_mid_plan_run(pe_mid_plan* plan, void* stream) {
 
 pe_status pe_mid_plan_results(pe_mid_plan* plan, pe_halfsupers (*results)[19], void* stream) {
   if (!plan || (plan->plan.nb && !results)) return fail(PE_ERR_INVALID, "null argument");
-  return mid_plan_results(plan->plan, results, static_cast<hipStream_t>(stream));
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  pe_status st = mid_plan_copy_results(plan->plan, results, s);
+  PE_HIP(hipStreamSynchronize(s), "mid plan results");
+  if (st == PE_OK) mid_plan_zero_tail(plan->plan, results);
+  return st;
 }
 
-pe_status pe_mid_plan_info(const pe_mid_plan* plan, int64_t* tasks, int64_t* workspace_bytes, pe_mid_config* config) {
+pe_status pe_mid_plan_info(const pe_mid_plan* plan, pe_mid_plan_stats* info, pe_mid_config* config) {
   if (!plan) return fail(PE_ERR_INVALID, "null plan");
-  if (tasks) *tasks = plan->plan.ntasks;
-  if (workspace_bytes) *workspace_bytes = plan->plan.arena_bytes;
+  const MidPlan& P = plan->plan;
+  if (info) {
+    std::memset(info, 0, sizeof *info);
+    info->small_boards = int64_t(P.layout.small.size());
+    info->large_boards = int64_t(P.layout.big.size());
+    info->launches = P.launches;
+    info->workspace_bytes = int64_t(P.layout.bytes);
+    info->workgroups = P.layout.workgroups;
+    info->streams = P.layout.streams;
+  }
   if (config) {
     std::memset(config, 0, sizeof *config);
-    config->workspace_budget = plan->plan.cfg.budget;
-    config->tile = plan->plan.cfg.tile;
-    config->workgroups = plan->plan.cfg.workgroups;
-    config->wait_limit_us = plan->wait_limit_us;
-    config->use_graph = plan->plan.cfg.graph;
+    config->workspace_budget = P.cfg.budget;
+    config->small_spots = P.cfg.small_spots;
+    config->workgroups = P.cfg.workgroups;
+    config->threads = P.cfg.threads;
+    config->streams = P.cfg.streams;
+    config->use_graph = P.cfg.graph;
   }
   return PE_OK;
 }
@@ -452,9 +476,8 @@ pe_status pe_mid_autotune(int n, const pe_high_board* boards, int64_t workspace_
   int dev = 0, cus = 0;
   PE_HIP(hipGetDevice(&dev), "hipGetDevice");
   PE_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute(CUs)");
-  const int per_cu = std::max(1, mid_queue_blocks_per_cu());
-  const int32_t tiles[] = {1024, 4096, 16384};
-  const int32_t wgs[] = {cus * std::max(1, per_cu / 2), cus * per_cu, cus * per_cu * 2};
+  const int32_t smalls[] = {12, 14, 16};
+  const int32_t threads[] = {256, 512, 1024};
   hipStream_t s = nullptr;
   PE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -462,38 +485,40 @@ pe_status pe_mid_autotune(int n, const pe_high_board* boards, int64_t workspace_
   (void)hipEventCreate(&e1);
   float best_ms = -1;
   pe_mid_config chosen{};
-  for (int32_t tile : tiles)
-    for (int32_t wg : wgs) {
+  for (int32_t sm : smalls)
+    for (int32_t th : threads) {
       pe_mid_config c{};
       c.workspace_budget = workspace_budget;
-      c.tile = tile;
-      c.workgroups = wg;
+      c.small_spots = sm;
+      c.threads = th;
+      c.workgroups = cus * mid_board_blocks_per_cu(th);
       MidConfig cfg;
       st = mid_resolve_config(&c, &cfg);
       if (st != PE_OK) break;
       MidPlan plan;
       st = mid_plan_create(infos, cfg, &plan);
-      if (st != PE_OK) break;
       float ms = 0;
-      // one warm run, one timed run
-      st = mid_plan_run(plan, s);
+      if (st == PE_OK) st = mid_plan_run(plan, s);   // warm-up
       if (st == PE_OK && hipEventRecord(e0, s) == hipSuccess) st = mid_plan_run(plan, s);
       if (st == PE_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                           hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
         st = fail(PE_ERR_DEVICE, "autotune timing failed");
-      int32_t err = 0;
-      if (st == PE_OK && hipMemcpy(&err, plan.error, sizeof err, hipMemcpyDeviceToHost) == hipSuccess && err)
-        st = fail(PE_ERR_DEVICE, "autotune: dependency wait exceeded the limit");
+      (void)hipStreamSynchronize(s);
       mid_plan_destroy(&plan);
       if (st != PE_OK) break;
-      if (best_ms < 0 || ms < best_ms) { best_ms = ms; chosen = c; }
+      if (best_ms < 0 || ms < best_ms) {
+        best_ms = ms;
+        chosen = c;
+        chosen.streams = cfg.streams;
+      }
     }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipStreamDestroy(s);
   if (st != PE_OK) return st;
   g_tuned[dev].set = true;
-  g_tuned[dev].tile = chosen.tile;
+  g_tuned[dev].small_spots = chosen.small_spots;
+  g_tuned[dev].threads = chosen.threads;
   g_tuned[dev].workgroups = chosen.workgroups;
   *best = chosen;
   return PE_OK;
@@ -539,12 +564,16 @@ pe_status pe_midsolve_batch(int n, const pe_high_board* boards, pe_halfsupers (*
   if (n < 0 || (n && (!boards || !results))) return fail(PE_ERR_INVALID, "bad batch");
   if (mode != 0 && mode != 1) return fail(PE_ERR_INVALID, "unsupported batch mode");
   if (mode == 1) {
-    pe_mid_plan* plan = nullptr;
-    pe_status st = pe_mid_plan_create(n, boards, nullptr, &plan);
+    std::vector<MidInfo> infos;
+    pe_status st = mid_infos(n, boards, &infos);
     if (st != PE_OK) return st;
-    st = pe_mid_plan_run(plan, stream);
-    if (st == PE_OK) st = pe_mid_plan_results(plan, results, stream);
-    pe_mid_plan_destroy(plan);
+    MidConfig cfg;
+    st = mid_resolve_config(nullptr, &cfg);
+    if (st != PE_OK) return st;
+    MidPlan plan;
+    st = mid_plan_create(infos, cfg, &plan);
+    if (st == PE_OK) st = mid_run_sync(plan, results, static_cast<hipStream_t>(stream));
+    mid_plan_destroy(&plan);
     return st;
   }
   std::vector<MidInfo> infos(n);
@@ -554,12 +583,24 @@ pe_status pe_midsolve_batch(int n, const pe_high_board* boards, pe_halfsupers (*
     bytes += mid_board_bytes(infos[b]);
   }
   const hipStream_t s = static_cast<hipStream_t>(stream);
-  void* arena = nullptr;
-  PE_HIP(hipMallocAsync(&arena, std::max<size_t>(bytes, 256), s), "hipMallocAsync(mid arena)");
+  // per-device scratch kept between calls (a caller's workspace, as midsolve's
+  // RawArray<halfsuper_s> workspace argument is), so a call costs launches, not allocations
+  int dev = 0;
+  PE_HIP(hipGetDevice(&dev), "hipGetDevice");
+  static std::mutex cache_mu[kMaxDevices];
+  static DeviceBuffer arena_cache[kMaxDevices], desc_cache[kMaxDevices];
+  std::lock_guard<std::mutex> lock(cache_mu[dev]);
+  PE_HIP(hipStreamSynchronize(s), "midsolve batch (stream idle)");
+  // large batches get a one-off arena so that the cache stays small (<= 8 GB)
+  const bool cached = bytes <= (size_t(8) << 30);
+  DeviceBuffer once;
+  struct FreeOnce { DeviceBuffer& b; ~FreeOnce() { if (b.ptr) (void)hipFree(b.ptr); } } free_once{once};
+  DeviceBuffer& ab = cached ? arena_cache[dev] : once;
+  PE_HIP(ab.reserve(std::max<size_t>(bytes, 256)), "hipMalloc(mid arena)");
+  void* arena = ab.ptr;
   std::vector<size_t> offs;
-  pe_status st = mid_solve_levels(infos, static_cast<uint8_t*>(arena), &offs, s);
+  pe_status st = mid_solve_levels(infos, static_cast<uint8_t*>(arena), &offs, s, &desc_cache[dev]);
   if (st == PE_OK) {
-    std::vector<pe_halfsupers> tmp(19);
     for (int b = 0; b < n; b++) {
       PE_HIP(hipMemcpyAsync(results[b], static_cast<uint8_t*>(arena) + offs[b], sizeof(pe_halfsupers) * 19,
                             hipMemcpyDeviceToHost, s), "hipMemcpyAsync(results)");
@@ -568,8 +609,7 @@ pe_status pe_midsolve_batch(int n, const pe_high_board* boards, pe_halfsupers (*
     for (int b = 0; b < n; b++)
       for (int i = infos[b].spots + 1; i < 19; i++) std::memset(&results[b][i], 0, sizeof(pe_halfsupers));
   }
-  PE_HIP(hipFreeAsync(arena, s), "hipFreeAsync(mid arena)");
-  PE_HIP(hipStreamSynchronize(s), "midsolve batch free");
+  PE_HIP(hipStreamSynchronize(s), "midsolve batch");
   return st;
 }
 

@@ -24,6 +24,22 @@ struct DevTables {
 pe_status device_tables(DevTables* out, std::string* err);
 
 void set_error(const std::string& e);
+
+// Grow-only device allocation reused across synchronous calls (callers serialize use and
+// have no work pending on it when it grows).
+struct DeviceBuffer {
+  void* ptr = nullptr;
+  size_t cap = 0;
+  hipError_t reserve(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    cap = 0;
+    const hipError_t e = hipMalloc(&ptr, bytes);
+    if (e == hipSuccess) cap = bytes;
+    return e;
+  }
+};
 #define PE_HIP(expr, msg)                                                           \
   do {                                                                              \
     hipError_t e_ = (expr);                                                         \

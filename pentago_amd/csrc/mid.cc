@@ -68,6 +68,7 @@ MidSizes mid_sizes(const MidInfo& I) {
   s.set0 = I.sets0_off[I.spots + 1];
   s.wins1 = I.wins1_off[I.spots + 1];
   s.cs1ps = I.cs1ps_off[I.spots + 1];
+  s.set1p = I.sets1p_off[I.spots + 1];
   s.work = I.workspace;
   return s;
 }

@@ -49,6 +49,7 @@ struct MidSizes {
   int64_t set0;      // Set0Info entries over all levels
   int64_t wins1;     // Wins1 entries
   int64_t cs1ps;     // uint16 entries
+  int64_t set1p;     // uint32 entries: the set s1p of each level as a bit mask over empty1 slots
   int64_t work;      // 32-byte workspace entries
 };
 MidSizes mid_sizes(const MidInfo& I);

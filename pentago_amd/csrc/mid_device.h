@@ -15,6 +15,7 @@ struct MidBoardDev {
   Set0Info* set0;
   Wins1* wins1;
   uint16_t* cs1ps;
+  uint32_t* set1p_bits;     // per level and s1p: bit q set for each member q of the subset
   uint64_t* work;           // 4 u64 per entry: win[2], notlose[2]
   pe_halfsupers* results;   // 19 entries
 };
@@ -104,12 +105,22 @@ __device__ inline void mid_prep_entry(const MidBoardDev& B, int64_t idx, const D
     (void)k0;
     const int parity = (n + I.ply) & 1;
     const uint64_t side1 = I.side[(spots + n + 1) & 1] | dside(I, k1, dsubset(C, spots, k1, s1));
-    const S4 w = super_wins(side1, T.superwin);
+    const S4 w = super_wins<true>(side1, T.superwin);
     const H2 after = split_half(w, parity), before = split_half(w, !parity);
     Wins1 out;
     out.after[0] = after.w[0]; out.after[1] = after.w[1];
     out.before[0] = before.w[0]; out.before[1] = before.w[1];
     B.wins1[e] = out;
+  } else if (idx >= n_set0 + n_wins1 + I.cs1ps_off[spots + 1]) {
+    // subset s1p of each level as a bit mask (the unranking of subsets.h:33-48, done once)
+    const int64_t e = idx - n_set0 - n_wins1 - I.cs1ps_off[spots + 1];
+    const int n = level_of(I.sets1p_off, spots, e);
+    const int s1p = int(e - I.sets1p_off[n]);
+    const int k0 = n >> 1, k1 = n - k0;
+    const uint64_t set1p = dsubset(C, spots - k0, k1, s1p);
+    uint32_t bits = 0;
+    for (int a = 0; a < k1; a++) bits |= 1u << (set1p >> (5 * a) & 0x1f);
+    B.set1p_bits[e] = bits;
   } else {
     // cs1ps (internal.h:133-148): remap s1p when a stone of the mover lands at empty1[j]
     const int64_t e = idx - n_set0 - n_wins1;
@@ -129,23 +140,28 @@ __device__ inline void mid_prep_entry(const MidBoardDev& B, int64_t idx, const D
 }
 
 // One position (internal.h:358-408), both aggressive passes at once.
-__device__ inline void mid_position(const MidBoardDev& B, int n, int64_t pos, const DevTables& T) {
+// A level holds fewer than 2^31 positions (at most 17,153,136 for 18 empty cells).
+__device__ inline void mid_position(const MidBoardDev& B, int n, int64_t pos64, uint64_t* __restrict__ work,
+                                    const DevTables& T) {
   const MidInfo& I = B.I;
   const int spots = I.spots;
-  const int k0 = n >> 1, k1 = n - k0;
+  const int k0 = n >> 1;
   const int32_t* C = T.choose;
-  const int nsets1p = dchoose(C, spots - k0, k1);
-  const int s0 = int(pos / nsets1p);
-  const int s1p = int(pos - int64_t(s0) * nsets1p);
-  const Set0Info& I0 = B.set0[I.sets0_off[n] + s0];
-  const uint64_t set1p = dsubset(C, spots - k0, k1, s1p);
-  uint32_t filled1p = 0;
+  const uint32_t pos = uint32_t(pos64);
+  const uint32_t nsets1p = uint32_t(I.sets1p_off[n + 1] - I.sets1p_off[n]);
+  const uint32_t s0 = pos / nsets1p;
+  const int s1p = int(pos - s0 * nsets1p);
+  const Set0Info& I0 = B.set0[I.sets0_off[n] + int(s0)];
+  const uint32_t filled1p = B.set1p_bits[I.sets1p_off[n] + s1p];
   uint16_t s1 = 0, s0p = 0;
-  for (int i = 0; i < k1; i++) {
-    const int q = int(set1p >> (5 * i) & 0x1f);
-    filled1p |= 1u << q;
-    s1 = uint16_t(s1 + dchoose(C, I0.empty1[q], i + 1));
-    s0p = uint16_t(s0p + I0.offset0[i * (spots - k0) + q]);
+  {
+    uint32_t rest = filled1p;
+    for (int i = 0; rest; i++) {   // members in increasing order: member i is q
+      const int q = __builtin_ctz(rest);
+      rest &= rest - 1;
+      s1 = uint16_t(s1 + C[I0.empty1[q] * 32 + i + 1]);
+      s0p = uint16_t(s0p + I0.offset0[i * (spots - k0) + q]);
+    }
   }
   uint64_t win0 = 0, win1 = 0, nl0 = 0, nl1 = 0;
   if (n == spots) nl0 = nl1 = ~uint64_t(0);
@@ -155,7 +171,7 @@ __device__ inline void mid_position(const MidBoardDev& B, int n, int64_t pos, co
   for (int m = 0; m < spots - n; m++) {
     const int i = __builtin_ctz(unmoved);
     unmoved &= unmoved - 1;
-    const uint64_t* c = B.work + 4 * (int64_t(gin.lo) + int64_t(I0.child_s0s[i]) * gin.ny + cs[i]);
+    const uint64_t* c = work + 4 * (int64_t(gin.lo) + int64_t(I0.child_s0s[i]) * gin.ny + cs[i]);
     const ulonglong2 a = reinterpret_cast<const ulonglong2*>(c)[0];
     const ulonglong2 b = reinterpret_cast<const ulonglong2*>(c)[1];
     win0 |= a.x; win1 |= a.y; nl0 |= b.x; nl1 |= b.y;
@@ -174,7 +190,7 @@ __device__ inline void mid_position(const MidBoardDev& B, int n, int64_t pos, co
   H2 nn; nn.w[0] = ~win0; nn.w[1] = ~win1;
   const H2 sw = hrmax(nw), sn = hrmax(nn);
   const MidGrab gout = I.spaces[n];
-  uint64_t* o = B.work + 4 * (int64_t(gout.lo) + int64_t(s1) * gout.ny + s0p);
+  uint64_t* o = work + 4 * (int64_t(gout.lo) + int64_t(s1) * gout.ny + s0p);
   reinterpret_cast<ulonglong2*>(o)[0] = make_ulonglong2(sw.w[0] | w1.before[0], sw.w[1] | w1.before[1]);
   reinterpret_cast<ulonglong2*>(o)[1] = make_ulonglong2(sn.w[0] | w1.before[0], sn.w[1] | w1.before[1]);
 }

@@ -45,7 +45,14 @@ __global__ __launch_bounds__(256) void mid_level_kernel(const MidBoardDev* __res
   if (t >= total) return;
   const int b = find_board(prefix, nb, t);
   const MidBoardDev& B = boards[b];
-  mid_position(B, B.I.spots - step, t - prefix[b], T);
+  mid_position(B, B.I.spots - step, t - prefix[b], B.work, T);
+}
+
+__global__ __launch_bounds__(256) void mid_level1_kernel(const MidBoardDev* __restrict__ board, int n,
+                                                         int64_t positions, DevTables T) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= positions) return;
+  mid_position(*board, n, t, board->work, T);
 }
 
 }  // namespace
@@ -53,27 +60,55 @@ __global__ __launch_bounds__(256) void mid_level_kernel(const MidBoardDev* __res
 // Allocation of one board's device scratch inside a caller-provided arena.
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
-size_t mid_board_bytes(const MidInfo& I) {
+size_t mid_table_bytes(const MidInfo& I) {
   const MidSizes s = mid_sizes(I);
   return align256(sizeof(Set0Info) * s.set0) + align256(sizeof(Wins1) * s.wins1) +
-         align256(sizeof(uint16_t) * s.cs1ps) + align256(32 * s.work) + align256(sizeof(pe_halfsupers) * 19);
+         align256(sizeof(uint16_t) * s.cs1ps) + align256(sizeof(uint32_t) * s.set1p);
+}
+size_t mid_work_bytes(const MidInfo& I) { return align256(32 * size_t(mid_sizes(I).work)); }
+size_t mid_board_bytes(const MidInfo& I) {
+  return mid_table_bytes(I) + mid_work_bytes(I) + align256(sizeof(pe_halfsupers) * 19);
 }
 
-void mid_carve(MidBoardDev* B, uint8_t* base) {
+void mid_carve_tables(MidBoardDev* B, uint8_t* base) {
   const MidSizes s = mid_sizes(B->I);
   size_t off = 0;
   B->set0 = reinterpret_cast<Set0Info*>(base + off); off += align256(sizeof(Set0Info) * s.set0);
   B->wins1 = reinterpret_cast<Wins1*>(base + off); off += align256(sizeof(Wins1) * s.wins1);
   B->cs1ps = reinterpret_cast<uint16_t*>(base + off); off += align256(sizeof(uint16_t) * s.cs1ps);
-  B->work = reinterpret_cast<uint64_t*>(base + off); off += align256(32 * s.work);
-  B->results = reinterpret_cast<pe_halfsupers*>(base + off);
+  B->set1p_bits = reinterpret_cast<uint32_t*>(base + off);
+}
+
+void mid_carve(MidBoardDev* B, uint8_t* base) {
+  mid_carve_tables(B, base);
+  const size_t t = mid_table_bytes(B->I);
+  B->work = reinterpret_cast<uint64_t*>(base + t);
+  B->results = reinterpret_cast<pe_halfsupers*>(base + t + mid_work_bytes(B->I));
+}
+
+pe_status mid_launch_level(const MidBoardDev* board, int n, int64_t positions, const DevTables& T,
+                           hipStream_t stream) {
+  if (positions <= 0) return PE_OK;
+  hipLaunchKernelGGL(mid_level1_kernel, dim3(unsigned((positions + 255) / 256)), dim3(256), 0, stream, board, n,
+                     positions, T);
+  PE_HIP(hipGetLastError(), "mid_level1_kernel launch");
+  return PE_OK;
+}
+
+pe_status mid_launch_prep(const MidBoardDev* boards, const int64_t* prefix, int nb, int64_t total,
+                          const DevTables& T, hipStream_t stream) {
+  if (total <= 0) return PE_OK;
+  hipLaunchKernelGGL(mid_prep_kernel, dim3(unsigned((total + 255) / 256)), dim3(256), 0, stream, boards, prefix, nb,
+                     total, T);
+  PE_HIP(hipGetLastError(), "mid_prep_kernel launch");
+  return PE_OK;
 }
 
 // Level-synchronous solve of many boards whose scratch lives in `arena` (device memory of
 // at least the sum of mid_board_bytes).  Results land in each board's scratch; the caller
 // copies them out (offsets returned in result_offsets).
 pe_status mid_solve_levels(const std::vector<MidInfo>& infos, uint8_t* arena, std::vector<size_t>* result_offsets,
-                           hipStream_t stream) {
+                           hipStream_t stream, DeviceBuffer* desc_cache) {
   DevTables T;
   std::string err;
   if (device_tables(&T, &err) != PE_OK) { set_error(err); return PE_ERR_DEVICE; }
@@ -95,7 +130,7 @@ pe_status mid_solve_levels(const std::vector<MidInfo>& infos, uint8_t* arena, st
   std::vector<int64_t> prefix(size_t(nb + 1) * (steps + 1), 0);
   for (int b = 0; b < nb; b++) {
     const MidSizes s = mid_sizes(infos[b]);
-    prefix[b + 1] = prefix[b] + s.set0 + s.wins1 + s.cs1ps;
+    prefix[b + 1] = prefix[b] + s.set0 + s.wins1 + s.cs1ps + s.set1p;
   }
   for (int t = 0; t < steps; t++) {
     int64_t* p = prefix.data() + size_t(nb + 1) * (t + 1);
@@ -110,7 +145,12 @@ pe_status mid_solve_levels(const std::vector<MidInfo>& infos, uint8_t* arena, st
   std::memcpy(blob.data(), boards.data(), desc_bytes);
   std::memcpy(blob.data() + desc_bytes, prefix.data(), sizeof(int64_t) * prefix.size());
   void* d = nullptr;
-  PE_HIP(hipMallocAsync(&d, bytes, stream), "hipMallocAsync(mid descriptors)");
+  if (desc_cache) {
+    PE_HIP(desc_cache->reserve(bytes), "hipMalloc(mid descriptors)");
+    d = desc_cache->ptr;
+  } else {
+    PE_HIP(hipMallocAsync(&d, bytes, stream), "hipMallocAsync(mid descriptors)");
+  }
   PE_HIP(hipMemcpyAsync(d, blob.data(), bytes, hipMemcpyHostToDevice, stream), "hipMemcpyAsync(mid descriptors)");
   const auto* dboards = static_cast<const MidBoardDev*>(d);
   const auto* dprefix = reinterpret_cast<const int64_t*>(static_cast<uint8_t*>(d) + desc_bytes);
@@ -129,7 +169,7 @@ pe_status mid_solve_levels(const std::vector<MidInfo>& infos, uint8_t* arena, st
                        dprefix + size_t(nb + 1) * (t + 1), nb, total, t, T);
     PE_HIP(hipGetLastError(), "mid_level_kernel launch");
   }
-  PE_HIP(hipFreeAsync(d, stream), "hipFreeAsync(mid descriptors)");
+  if (!desc_cache) PE_HIP(hipFreeAsync(d, stream), "hipFreeAsync(mid descriptors)");
   return PE_OK;
 }
 

@@ -1,4 +1,12 @@
-// Work-queue midsolve plans (host side); the kernels live in mid_queue.hip.
+// Batched midsolve plans (host side); the kernels live in mid_queue.hip.
+//
+// A batch is split by size.  Small boards (at most small_spots empty cells) are solved whole
+// by one workgroup each: a persistent kernel whose workgroups pop board indices from a device
+// queue (one atomic counter) and sweep every level of the board with workgroup barriers
+// between levels, in a workspace slot owned by the workgroup.  No workgroup ever waits for
+// another.  Large boards run level by level on one or more streams of their own (table pass,
+// then one launch per level over the whole device), boards of one stream reusing that
+// stream's slot in stream order.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -12,78 +20,64 @@ namespace pe {
 
 struct MidBoardDev;
 
-// Step codes of a task: the slot gate, the table-building pass, or a level step t >= 0
-// (level n = spots - t).
-constexpr int32_t kStepGate = -2;
-constexpr int32_t kStepPrep = -1;
-
-struct MidTask {
-  int32_t board, step;
-  int64_t begin;             // first entry (table entry or position index within the level)
-  int32_t count;             // entries in this task
-  int32_t dep_off, dep_cnt;  // dependency list: (counter index, required count) pairs
-  int32_t done_slot;         // counter incremented when the task has finished
-};
-static_assert(sizeof(MidTask) == 32, "task records are 32 bytes");
-
 struct MidConfig {
-  int64_t budget;            // bytes of workspace the batch may hold at once
-  int32_t tile;              // minimum entries per task
-  int32_t workgroups;        // persistent workgroups (256 threads each)
-  int64_t wait_limit;        // wall-clock ticks a dependency wait may last before failing
-  bool graph;                // replay the launch pair through a captured hipGraph
+  int64_t budget;        // bytes one run may hold (tables, workspaces, descriptors)
+  int32_t small_spots;   // boards with at most this many empty cells: one workgroup each
+  int32_t workgroups;    // persistent workgroups of the small-board kernel (upper bound)
+  int32_t threads;       // threads per small-board workgroup: 256, 512 or 1024
+  int32_t streams;       // streams carrying large boards (0: all work on the run stream)
+  bool graph;            // replay the whole launch sequence through a captured hipGraph
 };
+
+// Memory of one run, as offsets into a single device allocation.
+struct MidLayout {
+  std::vector<int> small, big;         // board indices (big: largest first)
+  std::vector<int> big_stream;         // stream of each big board (parallel to big)
+  int32_t workgroups = 0;              // small-board kernel grid, within the budget
+  int32_t streams = 0;                 // large-board streams (0: the run stream)
+  std::vector<size_t> table_off;       // per small board
+  size_t small_work = 0, small_stride = 0;   // workspace slot of each workgroup
+  size_t big_slots = 0, big_stride = 0;      // per stream: tables, then workspace
+  size_t bytes = 0;                    // total
+  // descriptor blob at offset 0: small boards | big boards | prep prefixes | results | head
+  size_t o_big = 0, o_prefix = 0, o_res = 0, o_head = 0, blob_bytes = 0;
+};
+// Returns null on success, else an error message.
+const char* mid_layout(const std::vector<MidInfo>& infos, const MidConfig& cfg, MidLayout* L);
 
 struct MidPlan {
   int device = -1;
   int nb = 0;
-  int64_t ntasks = 0;
-  int32_t nslots = 0;
   MidConfig cfg{};
-  std::vector<int> spots;
-  void* blob = nullptr;      // boards | tasks | deps | results | counters | head | error
-  void* arena = nullptr;     // workspace slots
-  int64_t arena_bytes = 0;
-  const MidBoardDev* boards = nullptr;
-  const MidTask* tasks = nullptr;
-  const int2* deps = nullptr;
-  pe_halfsupers* results = nullptr;
-  int32_t* counters = nullptr;
-  unsigned long long* head = nullptr;
-  int32_t* error = nullptr;
+  MidLayout layout;
+  std::vector<MidInfo> infos;
+  hipStream_t alloc_stream = nullptr;  // stream-ordered allocation (serving) or null
+  uint8_t* mem = nullptr;              // layout.bytes
+  const MidBoardDev* small_dev = nullptr;
+  const MidBoardDev* big_dev = nullptr;
+  const int64_t* prefix = nullptr;     // small boards' prep prefix, then [0, total] per big board
+  pe_halfsupers* results = nullptr;    // 19 per board, in board order
+  unsigned int* head = nullptr;
+  int64_t small_prep_total = 0;
+  std::vector<int64_t> big_prep_total;
   DevTables tables{};
+  std::vector<hipStream_t> streams;    // owned, layout.streams of them
+  std::vector<hipEvent_t> events;      // fork + one join per stream
   hipGraphExec_t graph_exec = nullptr;
+  int64_t launches = 0;                // kernel launches per run
 };
 
-// Order the tasks of a batch (simulated list schedule) and assign workspace slots within
-// cfg.budget, or use the given per-board arena offsets (every board resident at once).
-// Returns null on success, else an error message.
-const char* mid_plan_build_schedule(const std::vector<MidInfo>& infos, const MidConfig& cfg,
-                                    const std::vector<int64_t>* fixed_offsets, std::vector<MidTask>* order,
-                                    std::vector<int2>* deps, std::vector<int64_t>* offsets, int64_t* arena_bytes,
-                                    int32_t* nslots);
-
-// A schedule plus the layout of the plan's device blob
-// (boards | tasks | deps | results | counters | head, error).
-struct MidPrepared {
-  std::vector<MidTask> order;
-  std::vector<int2> deps;
-  std::vector<int64_t> offsets;
-  int64_t arena_bytes = 0;
-  int32_t nslots = 0;
-  size_t o_tasks = 0, o_deps = 0, o_res = 0, o_state = 0, o_head = 0, blob_bytes = 0;
-};
-pe_status mid_plan_prepare(const std::vector<MidInfo>& infos, const MidConfig& cfg,
-                           const std::vector<int64_t>* fixed_offsets, MidPrepared* p);
-// Host image of the blob (the first o_state bytes) for a blob at dev_blob and workspace at
-// arena; sets the plan's device pointers (blob and arena ownership stay with the caller).
-void mid_plan_fill(const std::vector<MidInfo>& infos, const MidConfig& cfg, const MidPrepared& p, void* dev_blob,
-                   uint8_t* arena, const DevTables& T, std::vector<uint8_t>* host_blob, MidPlan* plan);
-pe_status mid_plan_create(const std::vector<MidInfo>& infos, const MidConfig& cfg, MidPlan* plan);
-pe_status mid_plan_launch(const MidPlan& plan, hipStream_t stream);
+// alloc_stream null: synchronous hipMalloc/hipMemcpy; else stream-ordered on it.
+pe_status mid_plan_create(const std::vector<MidInfo>& infos, const MidConfig& cfg, MidPlan* plan,
+                          hipStream_t alloc_stream = nullptr);
 pe_status mid_plan_run(MidPlan& plan, hipStream_t stream);
-pe_status mid_plan_results(const MidPlan& plan, pe_halfsupers (*out)[19], hipStream_t stream);
+// Queue the copy of all results to host memory on `stream`; after the stream has been
+// synchronized, mid_plan_zero_tail clears the entries past each board's spots.
+pe_status mid_plan_copy_results(const MidPlan& plan, pe_halfsupers (*out)[19], hipStream_t stream);
+void mid_plan_zero_tail(const MidPlan& plan, pe_halfsupers (*out)[19]);
+// Frees the plan's memory (stream-ordered on alloc_stream when it was allocated that way).
 void mid_plan_destroy(MidPlan* plan);
+int mid_board_blocks_per_cu(int threads);
 
 // Serving loop (mid_server.cc)
 struct MidServer;
@@ -94,7 +88,19 @@ pe_status mid_server_wait(MidServer* S, int64_t ticket, pe_halfsupers* results, 
 void mid_server_stats(MidServer* S, pe_mid_server_info* out);
 void mid_server_destroy(MidServer* S);
 
-// Lay out one board's tables and workspace at `base` (mid_board_bytes(I) bytes).
+// Byte sizes of one board's tables, workspace, and both plus its 19 results.
+size_t mid_table_bytes(const MidInfo& I);
+size_t mid_work_bytes(const MidInfo& I);
+size_t mid_board_bytes(const MidInfo& I);
+// Lay out one board's tables (mid_table_bytes) at `base`; or tables, workspace and results
+// (mid_board_bytes) at `base`.
+void mid_carve_tables(MidBoardDev* B, uint8_t* base);
 void mid_carve(MidBoardDev* B, uint8_t* base);
+// One launch building the tables of every board: prefix[b] = prep entries before board b.
+pe_status mid_launch_prep(const MidBoardDev* boards, const int64_t* prefix, int nb, int64_t total,
+                          const DevTables& T, hipStream_t stream);
+// One launch sweeping level n of one board (all positions).
+pe_status mid_launch_level(const MidBoardDev* board, int n, int64_t positions, const DevTables& T,
+                           hipStream_t stream);
 
 }  // namespace pe

@@ -2,13 +2,13 @@
 // ones are still on the device (continuous batching).
 //
 // One worker thread owns the device work.  Each pass it (1) retires finished batches:
-// copies their results out, releases their workspace and marks the requests done, and
-// (2) while a stream is free, forms a batch from the pending requests in arrival order whose
-// workspace fits in the free part of the HBM pool (a request that does not fit waits; later,
-// smaller requests may pass it until it has waited kHeadOfLine, after which admission stops
-// until it fits), and launches the batch on that stream through the work-queue kernel.
-// Batches on different streams overlap on the device.  Each request's state (tables,
-// workspace, results) stays in device memory from admission to retirement.
+// copies their results out, releases their memory and marks the requests done, and (2)
+// while a stream is free, forms a batch from the pending requests in arrival order whose
+// device memory (tables, workspace, descriptors) fits in the part of the HBM budget not held
+// by batches in flight (a request that does not fit waits; later, smaller requests may pass
+// it until it has waited kHeadOfLine, after which admission stops until it fits), and runs
+// the batch on that stream as a batched plan.  Batches on different streams overlap on the
+// device.  Each request's state stays in device memory from admission to retirement.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -16,7 +16,6 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
-#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -29,60 +28,13 @@
 
 namespace pe {
 
-size_t mid_board_bytes(const MidInfo& I);
-int mid_queue_blocks_per_cu();
-
 namespace {
 
 using Clock = std::chrono::steady_clock;
 constexpr auto kHeadOfLine = std::chrono::milliseconds(20);
 
-// First-fit allocator over [0, size) with coalescing, 256-byte granules.
-class Pool {
- public:
-  explicit Pool(int64_t size) : size_(size) { free_[0] = size; }
-  int64_t alloc(int64_t bytes) {
-    bytes = (bytes + 255) & ~int64_t(255);
-    for (auto it = free_.begin(); it != free_.end(); ++it) {
-      if (it->second < bytes) continue;
-      const int64_t off = it->first, rem = it->second - bytes;
-      free_.erase(it);
-      if (rem) free_[off + bytes] = rem;
-      used_ += bytes;
-      peak_ = std::max(peak_, used_);
-      return off;
-    }
-    return -1;
-  }
-  void release(int64_t off, int64_t bytes) {
-    bytes = (bytes + 255) & ~int64_t(255);
-    used_ -= bytes;
-    auto it = free_.emplace(off, bytes).first;
-    auto next = std::next(it);
-    if (next != free_.end() && it->first + it->second == next->first) {
-      it->second += next->second;
-      free_.erase(next);
-    }
-    if (it != free_.begin()) {
-      auto prev = std::prev(it);
-      if (prev->first + prev->second == it->first) {
-        prev->second += it->second;
-        free_.erase(it);
-      }
-    }
-  }
-  int64_t size() const { return size_; }
-  int64_t peak() const { return peak_; }
-
- private:
-  int64_t size_, used_ = 0, peak_ = 0;
-  std::map<int64_t, int64_t> free_;
-};
-
 struct Request {
-  pe_high_board board;
   MidInfo info;
-  int64_t bytes = 0, offset = -1;
   Clock::time_point submitted, done_at;
   bool done = false;
   pe_status status = PE_OK;
@@ -93,9 +45,7 @@ struct Request {
 struct Batch {
   std::vector<int64_t> tickets;
   MidPlan plan;
-  void* blob = nullptr;
-  std::vector<uint8_t> host_blob;   // source of the blob upload, kept until the batch retires
-  int stream = -1;
+  int64_t bytes = 0;
 };
 
 }  // namespace
@@ -103,10 +53,8 @@ struct Batch {
 struct MidServer {
   int device = 0;
   int32_t max_batch = 512;
+  int64_t budget = 0, in_use = 0, peak = 0;
   MidConfig cfg{};
-  uint8_t* arena = nullptr;
-  Pool pool{0};
-  DevTables tables{};
   std::vector<hipStream_t> streams;
   std::vector<hipEvent_t> events;
   std::vector<pe_halfsupers*> staging;   // pinned, max_batch * 19 per stream
@@ -121,128 +69,108 @@ struct MidServer {
   bool stop = false;
   std::thread worker;
 
-  void fail_batch(Batch& b, pe_status st, const std::string& e) {
-    std::lock_guard<std::mutex> lock(mu);
-    for (int64_t t : b.tickets) {
-      Request& r = requests[t];
-      r.status = st;
-      r.error = e;
-      r.done = true;
-      r.done_at = Clock::now();
-      pool.release(r.offset, r.bytes);
-      completed++;
-    }
-    cv_done.notify_all();
-  }
-
-  // Launch the batch on its stream: upload, reset, kernel, results to pinned staging, event.
-  pe_status launch(Batch& b) {
-    std::vector<MidInfo> infos;
-    std::vector<int64_t> offsets;
-    {
-      std::lock_guard<std::mutex> lock(mu);
-      for (int64_t t : b.tickets) {
-        infos.push_back(requests[t].info);
-        offsets.push_back(requests[t].offset);
-      }
-    }
-    MidPrepared p;
-    pe_status st = mid_plan_prepare(infos, cfg, &offsets, &p);
-    if (st != PE_OK) return st;
-    const hipStream_t s = streams[size_t(b.stream)];
-    PE_HIP(hipMallocAsync(&b.blob, p.blob_bytes, s), "hipMallocAsync(batch)");
-    mid_plan_fill(infos, cfg, p, b.blob, arena, tables, &b.host_blob, &b.plan);
-    PE_HIP(hipMemcpyAsync(b.blob, b.host_blob.data(), b.host_blob.size(), hipMemcpyHostToDevice, s),
-           "hipMemcpyAsync(batch)");
-    st = mid_plan_launch(b.plan, s);
-    if (st != PE_OK) return st;
-    PE_HIP(hipMemcpyAsync(staging[size_t(b.stream)], b.plan.results, sizeof(pe_halfsupers) * 19 * infos.size(),
-                          hipMemcpyDeviceToHost, s), "hipMemcpyAsync(batch results)");
-    PE_HIP(hipMemcpyAsync(staging[size_t(b.stream)] + 19 * size_t(max_batch), b.plan.error, sizeof(int32_t),
-                          hipMemcpyDeviceToHost, s), "hipMemcpyAsync(batch error)");
-    PE_HIP(hipFreeAsync(b.blob, s), "hipFreeAsync(batch)");
-    PE_HIP(hipEventRecord(events[size_t(b.stream)], s), "hipEventRecord(batch)");
-    return PE_OK;
-  }
-
-  void retire(int k) {
+  // Finish the requests of a batch (results from staging when ok).  Caller holds no lock.
+  void finish(int k, pe_status st, const std::string& e) {
     Batch& b = batch_of_stream[size_t(k)];
     const pe_halfsupers* res = staging[size_t(k)];
-    int32_t err = 0;
-    std::memcpy(&err, res + 19 * size_t(max_batch), sizeof err);
     {
       std::lock_guard<std::mutex> lock(mu);
       const auto now = Clock::now();
       for (size_t i = 0; i < b.tickets.size(); i++) {
         Request& r = requests[b.tickets[i]];
-        std::memcpy(r.results, res + 19 * i, sizeof r.results);
-        for (int j = r.info.spots + 1; j < 19; j++) std::memset(&r.results[j], 0, sizeof(pe_halfsupers));
-        if (err) { r.status = PE_ERR_DEVICE; r.error = "dependency wait exceeded the wait limit"; }
+        if (st == PE_OK) {
+          std::memcpy(r.results, res + 19 * i, sizeof r.results);
+          for (int j = r.info.spots + 1; j < 19; j++) std::memset(&r.results[j], 0, sizeof(pe_halfsupers));
+        }
+        r.status = st;
+        r.error = e;
         r.done = true;
         r.done_at = now;
-        pool.release(r.offset, r.bytes);
         completed++;
       }
+      in_use -= b.bytes;
     }
     cv_done.notify_all();
+    mid_plan_destroy(&b.plan);   // stream-ordered free on the batch's stream
     b = Batch();
     stream_busy[size_t(k)] = 0;
+  }
+
+  // Launch the batch on stream k: plan (stream-ordered memory), run, results to staging.
+  pe_status launch(int k) {
+    Batch& b = batch_of_stream[size_t(k)];
+    const hipStream_t s = streams[size_t(k)];
+    std::vector<MidInfo> infos;
+    {
+      std::lock_guard<std::mutex> lock(mu);
+      for (int64_t t : b.tickets) infos.push_back(requests[t].info);
+    }
+    pe_status st = mid_plan_create(infos, cfg, &b.plan, s);
+    if (st != PE_OK) return st;
+    st = mid_plan_run(b.plan, s);
+    if (st != PE_OK) return st;
+    st = mid_plan_copy_results(b.plan, reinterpret_cast<pe_halfsupers (*)[19]>(staging[size_t(k)]), s);
+    if (st != PE_OK) return st;
+    PE_HIP(hipEventRecord(events[size_t(k)], s), "hipEventRecord(batch)");
+    return PE_OK;
+  }
+
+  // Bytes a batch of these requests would hold.
+  int64_t batch_bytes(const std::vector<MidInfo>& infos) const {
+    MidConfig c = cfg;
+    c.budget = int64_t(1) << 62;
+    MidLayout L;
+    if (mid_layout(infos, c, &L)) return int64_t(1) << 62;
+    return int64_t(L.bytes);
   }
 
   void run() {
     (void)hipSetDevice(device);
     for (;;) {
       bool progressed = false;
-      // retire finished batches
       for (size_t k = 0; k < streams.size(); k++) {
         if (!stream_busy[k]) continue;
         const hipError_t q = hipEventQuery(events[k]);
         if (q == hipErrorNotReady) continue;
-        if (q != hipSuccess) {
-          fail_batch(batch_of_stream[k], PE_ERR_DEVICE, std::string("batch failed: ") + hipGetErrorString(q));
-          batch_of_stream[k] = Batch();
-          stream_busy[k] = 0;
-        } else {
-          retire(int(k));
-        }
+        if (q != hipSuccess) finish(int(k), PE_ERR_DEVICE, std::string("batch failed: ") + hipGetErrorString(q));
+        else finish(int(k), PE_OK, "");
         progressed = true;
       }
-      // admit pending requests onto free streams
       for (size_t k = 0; k < streams.size(); k++) {
         if (stream_busy[k]) continue;
         Batch b;
-        b.stream = int(k);
+        std::vector<MidInfo> infos;
         {
           std::lock_guard<std::mutex> lock(mu);
           const auto now = Clock::now();
-          bool head_blocked = false;
           for (auto it = pending.begin(); it != pending.end() && int32_t(b.tickets.size()) < max_batch;) {
             Request& r = requests[*it];
-            const int64_t off = head_blocked ? -1 : pool.alloc(r.bytes);
-            if (off < 0) {
+            infos.push_back(r.info);
+            const int64_t bytes = batch_bytes(infos);
+            if (in_use + bytes > budget) {
+              infos.pop_back();
               // the oldest waiting request blocks further admission once it has waited long
-              if (it == pending.begin() && now - r.submitted > kHeadOfLine) head_blocked = true;
+              if (it == pending.begin() && now - r.submitted > kHeadOfLine) break;
               ++it;
               continue;
             }
-            r.offset = off;
+            b.bytes = bytes;
             b.tickets.push_back(*it);
             it = pending.erase(it);
+          }
+          if (!b.tickets.empty()) {
+            in_use += b.bytes;
+            peak = std::max(peak, in_use);
+            batches++;
           }
         }
         if (b.tickets.empty()) break;
         batch_of_stream[k] = std::move(b);
         stream_busy[k] = 1;
-        const pe_status st = launch(batch_of_stream[k]);
+        const pe_status st = launch(int(k));
         if (st != PE_OK) {
-          // the stream may hold part of the batch: wait for it before reusing memory
           (void)hipStreamSynchronize(streams[k]);
-          fail_batch(batch_of_stream[k], st, "batch launch failed");
-          batch_of_stream[k] = Batch();
-          stream_busy[k] = 0;
-        } else {
-          std::lock_guard<std::mutex> lock(mu);
-          batches++;
+          finish(int(k), st, "batch launch failed");
         }
         progressed = true;
       }
@@ -258,14 +186,14 @@ struct MidServer {
 };
 
 pe_status mid_server_create(const pe_mid_server_config* c, MidServer** out) {
-  std::string err;
   auto* S = new MidServer;
   auto fail = [&](pe_status st, const std::string& e) { set_error(e); delete S; return st; };
-  if (device_tables(&S->tables, &err) != PE_OK) return fail(PE_ERR_DEVICE, err);
+  DevTables T;
+  std::string err;
+  if (device_tables(&T, &err) != PE_OK) return fail(PE_ERR_DEVICE, err);
   if (hipGetDevice(&S->device) != hipSuccess) return fail(PE_ERR_DEVICE, "hipGetDevice failed");
-  int cus = 0, rate_khz = 0;
+  int cus = 0;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, S->device);
-  (void)hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, S->device);
   int64_t budget = c ? c->workspace_budget : 0;
   if (budget <= 0) {
     size_t free_b = 0, total_b = 0;
@@ -274,22 +202,27 @@ pe_status mid_server_create(const pe_mid_server_config* c, MidServer** out) {
   }
   const int nstreams = c && c->streams > 0 ? c->streams : 4;
   S->max_batch = c && c->max_batch > 0 ? c->max_batch : 512;
+  S->budget = budget;
   S->cfg.budget = budget;
-  S->cfg.tile = c && c->tile > 0 ? c->tile : 1024;
-  S->cfg.workgroups = c && c->workgroups > 0 ? c->workgroups : cus * std::max(1, mid_queue_blocks_per_cu());
-  S->cfg.wait_limit = int64_t(10) * 1000 * int64_t(std::max(rate_khz, 1));   // 10 s
+  S->cfg.small_spots = c && c->small_spots > 0 ? c->small_spots : 14;
+  S->cfg.threads = 512;
+  S->cfg.workgroups = c && c->workgroups > 0 ? c->workgroups : cus * mid_board_blocks_per_cu(512) / nstreams;
+  S->cfg.workgroups = std::max(S->cfg.workgroups, 1);
+  S->cfg.streams = 0;   // a batch runs entirely on its server stream
   S->cfg.graph = false;
-  void* arena = nullptr;
-  if (hipMalloc(&arena, size_t(budget)) != hipSuccess) return fail(PE_ERR_DEVICE, "hipMalloc(server workspace) failed");
-  S->arena = static_cast<uint8_t*>(arena);
-  S->pool = Pool(budget);
+  // keep freed stream-ordered memory cached for the next batches
+  hipMemPool_t pool = nullptr;
+  if (hipDeviceGetDefaultMemPool(&pool, S->device) == hipSuccess) {
+    uint64_t threshold = uint64_t(budget);
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
+  }
   for (int k = 0; k < nstreams; k++) {
     hipStream_t s = nullptr;
     hipEvent_t e = nullptr;
     void* h = nullptr;
     if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess ||
-        hipHostMalloc(&h, sizeof(pe_halfsupers) * 19 * size_t(S->max_batch) + 64, hipHostMallocDefault) != hipSuccess)
+        hipHostMalloc(&h, sizeof(pe_halfsupers) * 19 * size_t(S->max_batch), hipHostMallocDefault) != hipSuccess)
       return fail(PE_ERR_DEVICE, "server stream setup failed");
     S->streams.push_back(s);
     S->events.push_back(e);
@@ -304,10 +237,8 @@ pe_status mid_server_create(const pe_mid_server_config* c, MidServer** out) {
 
 pe_status mid_server_submit(MidServer* S, const pe_high_board& board, int64_t* ticket) {
   Request r;
-  r.board = board;
   if (!make_mid_info(board, &r.info)) { set_error("not a valid board with >= 18 stones"); return PE_ERR_INVALID; }
-  r.bytes = int64_t(mid_board_bytes(r.info));
-  if (r.bytes > S->pool.size()) { set_error("board needs more workspace than the server holds"); return PE_ERR_CAPACITY; }
+  if (S->batch_bytes({r.info}) > S->budget) { set_error("board needs more workspace than the server holds"); return PE_ERR_CAPACITY; }
   {
     std::lock_guard<std::mutex> lock(S->mu);
     r.submitted = Clock::now();
@@ -358,7 +289,7 @@ void mid_server_stats(MidServer* S, pe_mid_server_info* out) {
   out->submitted = S->submitted;
   out->completed = S->completed;
   out->batches = S->batches;
-  out->peak_workspace = S->pool.peak();
+  out->peak_workspace = S->peak;
   out->pending = int64_t(S->pending.size());
   out->in_flight = S->submitted - S->completed - int64_t(S->pending.size());
 }
@@ -376,7 +307,6 @@ void mid_server_destroy(MidServer* S) {
     (void)hipEventDestroy(S->events[k]);
     (void)hipHostFree(S->staging[k]);
   }
-  if (S->arena) (void)hipFree(S->arena);
   delete S;
 }
 

@@ -223,7 +223,16 @@ PE_HD S4 reduce3(S4 a) {
 
 PE_HD S4 load_s4(const uint64_t* p) { S4 r; for (int i = 0; i < 4; i++) r.w[i] = p[i]; return r; }
 
-// table: 4*512*5 supers as u64 words, entry ((q*512+state)*5+field)*4
+// Compiler-only memory fence: keeps the table loads of later pattern groups from being
+// hoisted above earlier groups, which would keep dozens of loaded words live at once.
+#define PE_GROUP_FENCE() \
+  do {                   \
+    if (kLean) asm volatile("" ::: "memory"); \
+  } while (0)
+
+// table: 4*512*5 supers as u64 words, entry ((q*512+state)*5+field)*4.  kLean evaluates the
+// pattern groups one after another (fewer live registers, less overlap of table loads).
+template <bool kLean = false>
 PE_HD S4 super_wins(uint64_t side, const uint64_t* table) {
   const uint64_t* t0 = table + size_t((0 * 512 + (side & 0x1ff)) * 5) * 4;
   const uint64_t* t1 = table + size_t((1 * 512 + (side >> 16 & 0x1ff)) * 5) * 4;
@@ -232,12 +241,19 @@ PE_HD S4 super_wins(uint64_t side, const uint64_t* table) {
   enum { H = 0, V = 4, LO = 8, HI = 12, AS = 16 };
   S4 wins = s4_zero();
   wins = s4_or(wins, reduce2(reduce3(s4_and(load_s4(t0 + V), load_s4(t1 + V)))));
+  PE_GROUP_FENCE();
   wins = s4_or(wins, reduce1(reduce3(s4_and(load_s4(t0 + H), load_s4(t2 + H)))));
+  PE_GROUP_FENCE();
   wins = s4_or(wins, reduce2(reduce0(s4_and(load_s4(t1 + H), load_s4(t3 + H)))));
+  PE_GROUP_FENCE();
   wins = s4_or(wins, reduce1(reduce0(s4_and(load_s4(t2 + V), load_s4(t3 + V)))));
+  PE_GROUP_FENCE();
   wins = s4_or(wins, reduce1(s4_and(s4_and(load_s4(t0 + LO), load_s4(t2 + AS)), load_s4(t3 + LO))));
+  PE_GROUP_FENCE();
   wins = s4_or(wins, reduce2(s4_and(s4_and(load_s4(t0 + HI), load_s4(t1 + AS)), load_s4(t3 + HI))));
+  PE_GROUP_FENCE();
   wins = s4_or(wins, reduce3(s4_and(s4_and(load_s4(t1 + LO), load_s4(t0 + AS)), load_s4(t2 + LO))));
+  PE_GROUP_FENCE();
   wins = s4_or(wins, reduce0(s4_and(s4_and(load_s4(t1 + HI), load_s4(t3 + AS)), load_s4(t2 + HI))));
   return wins;
 }
@@ -265,7 +281,7 @@ PE_HD H2 split_half(S4 s, int parity) {
   return h;
 }
 PE_HD H2 halfsuper_wins(uint64_t side, int parity, const uint64_t* table) {
-  return split_half(super_wins(side, table), parity);
+  return split_half(super_wins<true>(side, table), parity);
 }
 // rmax on halves; flips parity (pentago/mid/halfsuper.h:18-42, 155-203).
 PE_HD uint64_t hrmax_lo(uint64_t x) {

@@ -1,6 +1,6 @@
-"""The persistent work-queue midsolve (pe_mid_plan_*, pe_midsolve_batch mode 1) against
-the level-synchronous batch and the oracle: mixed sizes, tight workspace budgets that force
-slot reuse, graph replays, the autotuner and the bounded-wait error path."""
+"""Batched midsolve plans (pe_mid_plan_*, pe_midsolve_batch mode 1) against the plain
+per-board solve and the oracle: mixed sizes, every split between the small-board queue
+kernel and the large-board launches, tight budgets, graph replays and the autotuner."""
 import numpy as np
 import pytest
 
@@ -62,19 +62,30 @@ def test_batch_mode1_matches_mode0():
     assert P.midsolve_batch(boards, mode=1) == P.midsolve_batch(boards, mode=0)
 
 
-@pytest.mark.parametrize("slots", [1, 3])
-def test_queue_tight_budget_reuses_workspace(slots):
-    """A budget of one or three of the largest boards: boards wait for slots (gate chains)."""
-    boards = mixed_boards(5, 60, 22, 30)
-    big = max(board_spots(b) for b in boards)
-    from tests.test_mid_schedule import board_bytes_upper
-    budget = int(1.3 * slots * board_bytes_upper(big)) + (64 << 20)
-    got = solve_queue(boards, workspace_budget=budget, tile=512, workgroups=128)
+@pytest.mark.parametrize("small_spots,threads,streams", [(4, 256, 1), (10, 1024, 2), (14, 512, -1), (18, 256, 3)])
+def test_every_split_matches_plain(small_spots, threads, streams):
+    boards = mixed_boards(7, 50, 22, 32)
+    got = solve_queue(boards, small_spots=small_spots, threads=threads, streams=streams)
+    assert got == [P.midsolve_internal(b) for b in boards]
+
+
+def test_tight_budget_shrinks_plan():
+    """A budget of one small-board workspace and one large-board slot (plus tables)."""
+    boards = mixed_boards(5, 60, 20, 30)
+    from tests.test_mid_schedule import expected_bytes
+    budget = expected_bytes(boards, 14, 1, 1) + (2 << 20)
+    plan = P.MidPlan(boards, workspace_budget=budget, small_spots=14, streams=4)
+    info = plan.info()
+    print("plan", info, flush=True)
+    assert info["workspace_bytes"] <= budget and info["streams"] <= 1
+    plan.run()
+    got = P.results_to_lists(boards, plan.results())
+    plan.close()
     assert got == P.midsolve_batch(boards, mode=0)
 
 
 def test_graph_replays_match_eager():
-    boards = mixed_boards(6, 40, 23, 32)
+    boards = mixed_boards(6, 40, 20, 32)
     expect = P.midsolve_batch(boards, mode=0)
     plan = P.MidPlan(boards, use_graph=True)
     for rep in range(3):
@@ -88,22 +99,10 @@ def test_graph_replays_match_eager():
 def test_autotune_picks_a_candidate_and_becomes_default():
     boards = seeded_boards(48, 10, 22, 30)
     best = P.mid_autotune(boards)
-    assert best["tile"] in (1024, 4096, 16384) and best["workgroups"] > 0
+    assert best["small_spots"] in (12, 14, 16) and best["threads"] in (256, 512, 1024)
     plan = P.MidPlan(boards)
     info = plan.info()
-    assert info["tile"] == best["tile"] and info["workgroups"] == best["workgroups"]
+    assert (info["small_spots"], info["threads"]) == (best["small_spots"], best["threads"])
     plan.run()
     assert P.results_to_lists(boards, plan.results()) == P.midsolve_batch(boards, mode=0)
     plan.close()
-
-
-def test_wait_limit_reports_error_and_queue_recovers():
-    """A 1 us bound on every dependency wait: the kernel reports the failed wait, skips the
-    remaining work and still exits; the next plan runs normally."""
-    boards = seeded_boards(32, 12, 20, 26)
-    plan = P.MidPlan(boards, wait_limit_us=1, workgroups=512, tile=256)
-    plan.run()
-    with pytest.raises(P.PentagoError):
-        plan.results()
-    plan.close()
-    assert solve_queue(boards[:8]) == P.midsolve_batch(boards[:8], mode=0)

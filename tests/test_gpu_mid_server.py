@@ -49,8 +49,8 @@ def test_same_alone_and_in_the_mix():
 def test_pool_budget_makes_requests_wait():
     boards = seeded_boards(40, 32, 20, 30)
     big = max(board_spots(b) for b in boards)
-    from tests.test_mid_schedule import board_bytes_upper
-    budget = int(1.5 * board_bytes_upper(big)) + (32 << 20)
+    from tests.test_mid_schedule import table_bytes, work_bytes
+    budget = int(1.5 * (table_bytes(big) + work_bytes(big))) + (4 << 20)
     server = P.MidServer(workspace_budget=budget, streams=2, max_batch=8)
     tickets = [server.submit(b) for b in boards]
     got = [server.wait(t)[0] for t in tickets]

@@ -1,52 +1,81 @@
-"""Host-side checks of the midsolve work-queue schedule (no GPU): the task order respects
-every dependency, every step's tasks are present, boards sharing a workspace slot are
-chained through their gates and the workspace stays within the budget.  The library
-checks these invariants itself while building the order; these tests drive it over the
-batched and serving board mixes and over tight budgets."""
+"""Host-side checks of batched midsolve plans (no GPU): how a batch is split between the
+small-board kernel and the large-board launches, how much device memory it needs, and that
+tight budgets shrink the plan or are refused."""
 import pytest
 
 import pentago_amd as P
 from pentago_amd.workloads import board_spots, seeded_boards
 
 
-def board_bytes_upper(spots):
-    # 32-byte workspace entries dominate; generous bound used only to size budgets
+def work_bytes(spots):
+    """32-byte workspace entries: the larger of two adjacent levels, summed (bottleneck)."""
     from math import comb
     worst, prev = 0, 1
     for n in range(spots + 1):
         nxt = comb(spots, n + 1) * comb(n + 1, (n + 1) // 2) if n + 1 <= spots else 0
         worst = max(worst, prev + nxt)
         prev = nxt
-    return 32 * worst
+    return (32 * worst + 255) & ~255
 
 
-def test_schedule_batched_mix():
+def table_bytes(spots):
+    """Bytes of one board's tables (set0_info, wins1, cs1ps, subset masks), 256-aligned."""
+    from math import comb
+    up = lambda x: (x + 255) & ~255
+    set0 = sum(comb(spots, n // 2) for n in range(spots + 1))
+    wins1 = sum(comb(spots, n - n // 2) for n in range(spots + 1))
+    s1p = [comb(spots - n // 2, n - n // 2) for n in range(spots + 1)]
+    cs1ps = sum(c * (spots - n // 2) for n, c in enumerate(s1p))
+    return up(256 * set0) + up(32 * wins1) + up(2 * cs1ps) + up(4 * sum(s1p))
+
+
+def expected_bytes(boards, small_spots, workgroups, streams):
+    """Device memory of a plan: descriptors, small tables, one workspace per workgroup, one
+    tables+workspace slot per large-board stream (descriptor part bounded from above)."""
+    small = [board_spots(b) for b in boards if board_spots(b) <= small_spots]
+    big = [board_spots(b) for b in boards if board_spots(b) > small_spots]
+    wgs = min(workgroups, len(small))
+    fixed = sum(table_bytes(s) for s in small)
+    slots = wgs * (max(work_bytes(s) for s in small) if small else 0)
+    bigs = (max(streams, 1) if big else 0) * (max(table_bytes(s) + work_bytes(s) for s in big) if big else 0)
+    return fixed + slots + bigs
+
+
+def test_plan_memory_matches_the_layout():
     boards = seeded_boards(512, 7, 18, 30)
-    tasks, ws = P.mid_schedule_info(boards, int(200e9), 4096, 2048)
-    assert tasks > len(boards) * 3
-    assert 0 < ws <= int(200e9)
+    launches, ws = P.mid_schedule_info(boards, int(200e9), small_spots=14, workgroups=1024, threads=512, streams=2)
+    est = expected_bytes(boards, 14, 1024, 2)
+    assert est <= ws <= est + 4096 * len(boards) + (1 << 20)      # + descriptors
+    big = [b for b in boards if board_spots(b) > 14]
+    assert launches == 3 + sum(2 + board_spots(b) for b in big)
 
 
-@pytest.mark.parametrize("budget_boards", [1, 2, 5])
-def test_schedule_tight_budget_reuses_slots(budget_boards):
-    """A budget of a few of the largest boards forces slot reuse (gate chains)."""
-    boards = seeded_boards(200, 11, 22, 30)
-    big = max(board_spots(b) for b in boards)
-    budget = int(1.3 * budget_boards * board_bytes_upper(big)) + (64 << 20)
-    tasks, ws = P.mid_schedule_info(boards, budget, 1024, 512)
-    assert ws <= budget
+@pytest.mark.parametrize("small_spots", [10, 14, 18])
+def test_split_by_size(small_spots):
+    boards = seeded_boards(200, 11, 18, 34)
+    launches, ws = P.mid_schedule_info(boards, int(200e9), small_spots=small_spots, workgroups=256, streams=1)
+    big = [b for b in boards if board_spots(b) > small_spots]
+    small = [b for b in boards if board_spots(b) <= small_spots]
+    assert launches == (3 if small else 0) + sum(2 + board_spots(b) for b in big)
 
 
-def test_schedule_rejects_board_over_budget():
-    boards = seeded_boards(4, 3, 18, 18)
+def test_tight_budget_shrinks_the_plan():
+    """A budget just above the minimum (one workspace slot, one large-board slot) is
+    accepted with fewer workgroups and streams; anything below it is refused."""
+    boards = seeded_boards(200, 12, 20, 32)
+    small = [b for b in boards if board_spots(b) <= 14]
+    minimum = expected_bytes(boards, 14, 1, 1)
+    launches, ws = P.mid_schedule_info(boards, minimum + (2 << 20), small_spots=14, workgroups=1024, streams=4)
+    assert ws <= minimum + (2 << 20)
     with pytest.raises(P.PentagoError):
-        P.mid_schedule_info(boards, 1 << 20, 4096, 256)
+        P.mid_schedule_info(boards, minimum - work_bytes(max(board_spots(b) for b in small)) // 2, small_spots=14,
+                            workgroups=1024, streams=4)
 
 
 def test_schedule_rejects_incomplete_config():
     boards = seeded_boards(2, 3, 26, 26)
     with pytest.raises(P.PentagoError):
-        P.mid_schedule_info(boards, 0, 4096, 256)
+        P.mid_schedule_info(boards, 0)
 
 
 def test_seeded_boards_follow_the_rules():
