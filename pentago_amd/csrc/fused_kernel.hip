@@ -98,18 +98,17 @@ void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __r
                          uint64_t* __restrict__ output) {
   const BlockDev& B = blocks[blockIdx.x / kParts];
   const SecDev& S = secs[B.sec];
-  const int p = (blockIdx.x % kParts) * kThreads + threadIdx.x;
+  // wave w of the block holds (loc0, loc1) = (w / 8, w % 8) and its 64 lanes (loc2, loc3):
+  // quadrants 0 and 1 are the same for the whole wave, so their table reads are scalar
+  // loads and their partial products scalar operations.  Lanes outside a partial edge
+  // block idle.
+  const int wave = (blockIdx.x % kParts) * (kThreads / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   const int bs0 = min(8, S.rmin_size[0] - 8 * B.b[0]), bs1 = min(8, S.rmin_size[1] - 8 * B.b[1]);
   const int bs2 = min(8, S.rmin_size[2] - 8 * B.b[2]), bs3 = min(8, S.rmin_size[3] - 8 * B.b[3]);
-  if (p >= bs0 * bs1 * bs2 * bs3) return;
-  int loc[4];
-  {
-    int rem = p;
-    loc[3] = rem % bs3; rem /= bs3;
-    loc[2] = rem % bs2; rem /= bs2;
-    loc[1] = rem % bs1;
-    loc[0] = rem / bs1;
-  }
+  int loc[4] = {wave >> 3, wave & 7, lane >> 3, lane & 7};
+  if (loc[0] >= bs0 || loc[1] >= bs1 || loc[2] >= bs2 || loc[3] >= bs3) return;
+  const int p = ((loc[0] * bs1 + loc[1]) * bs2 + loc[2]) * bs3 + loc[3];
 
   // ---- children: per open dimension, AND the moves' children shifted along the child
   //      dimension (OR of complements = complement of AND), then apply the rest of the
