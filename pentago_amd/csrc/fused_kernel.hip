@@ -135,15 +135,28 @@ void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __r
     const uint64_t* in = L.input + 8 * L.in_base;
     S4 acc0, acc1;
     for (int i = 0; i < 4; i++) acc0.w[i] = acc1.w[i] = ~uint64_t(0);
-    for (int m = 0; m < nm; m++) {
-      const int e = row[m];
-      const int kk = e & 63;
-      const int64_t cidx = int64_t(kk) * L.in_kstride + (kk == last ? base_last : base) +
-                           int64_t(L.input_stride) * ((e >> 6) & 7);
-      const uint64_t* c = in + 8 * cidx;
-      const int k = e >> 9;
-      acc0 = s4_and(acc0, shift_coord_var(ld(c), cd, k));
-      acc1 = s4_and(acc1, shift_coord_var(ld(c + 4), cd, k));
+    // moves in groups of three: all loads of a group are issued before any is used
+    for (int m0 = 0; m0 < nm; m0 += 3) {
+      S4 c0[3], c1[3];
+      int ks[3];
+#pragma unroll
+      for (int u = 0; u < 3; u++) {
+        if (m0 + u >= nm) break;
+        const int e = row[m0 + u];
+        const int kk = e & 63;
+        const int64_t cidx = int64_t(kk) * L.in_kstride + (kk == last ? base_last : base) +
+                             int64_t(L.input_stride) * ((e >> 6) & 7);
+        const uint64_t* c = in + 8 * cidx;
+        ks[u] = e >> 9;
+        c0[u] = ld(c);
+        c1[u] = ld(c + 4);
+      }
+#pragma unroll
+      for (int u = 0; u < 3; u++) {
+        if (m0 + u >= nm) break;
+        acc0 = s4_and(acc0, shift_coord_var(c0[u], cd, ks[u]));
+        acc1 = s4_and(acc1, shift_coord_var(c1[u], cd, ks[u]));
+      }
     }
     const int y = local_add(L.inv_local, local_sym);
     t0 = s4_or(t0, rearrange(shift_local(s4_not(acc0), y), L.inv_global));
