@@ -95,14 +95,21 @@ __global__ __launch_bounds__(kThreads)
 void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __restrict__ lines,
                          const SecDev* __restrict__ secs, const uint64_t* __restrict__ fields,
                          const uint16_t* __restrict__ movetab, const uint8_t* __restrict__ refl,
-                         uint64_t* __restrict__ output) {
-  const BlockDev& B = blocks[blockIdx.x / kParts];
+                         uint64_t* __restrict__ output, int nblocks, int blocks_per_xcd) {
+  // Workgroups are dispatched to the 8 XCDs round-robin: workgroup i runs on XCD i % 8.
+  // XCD x takes the contiguous run of blocks [x * blocks_per_xcd, ...), so the blocks that
+  // share child lines (neighbours along the last axes) and the two halves of a 128-byte line
+  // meet in the same L2.
+  const int xcd = blockIdx.x & 7, r = blockIdx.x >> 3;
+  const int blk = xcd * blocks_per_xcd + r / kParts, part = r % kParts;
+  if (r / kParts >= blocks_per_xcd || blk >= nblocks) return;
+  const BlockDev& B = blocks[blk];
   const SecDev& S = secs[B.sec];
   // wave w of the block holds (loc0, loc1) = (w / 8, w % 8) and its 64 lanes (loc2, loc3):
   // quadrants 0 and 1 are the same for the whole wave, so their table reads are scalar
   // loads and their partial products scalar operations.  Lanes outside a partial edge
   // block idle.
-  const int wave = (blockIdx.x % kParts) * (kThreads / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wave = part * (kThreads / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int bs0 = min(8, S.rmin_size[0] - 8 * B.b[0]), bs1 = min(8, S.rmin_size[1] - 8 * B.b[1]);
   const int bs2 = min(8, S.rmin_size[2] - 8 * B.b[2]), bs3 = min(8, S.rmin_size[3] - 8 * B.b[3]);
@@ -380,13 +387,15 @@ pe_status fused_plan_run(const FusedPlan& plan, hipStream_t stream) {
   std::string err;
   if (device_tables(&T, &err) != PE_OK) { set_error(err); return PE_ERR_DEVICE; }
   auto* base = static_cast<uint8_t*>(plan.blob);
-  const int64_t grid = int64_t(plan.nblocks) * kParts;
+  const int per_xcd = (plan.nblocks + 7) / 8;
+  const int64_t grid = int64_t(per_xcd) * kParts * 8;
   if (grid > 0x7fffffff) { set_error("fused plan too large"); return PE_ERR_CAPACITY; }
   hipLaunchKernelGGL(fused_blocks_kernel, dim3(unsigned(grid)), dim3(kThreads), 0, stream,
                      reinterpret_cast<const BlockDev*>(base), reinterpret_cast<const LineDev*>(base + plan.off_lines),
                      reinterpret_cast<const SecDev*>(base + plan.off_secs),
                      reinterpret_cast<const uint64_t*>(base + plan.off_fields),
-                     reinterpret_cast<const uint16_t*>(base + plan.off_moves), base + plan.off_refl, plan.output);
+                     reinterpret_cast<const uint16_t*>(base + plan.off_moves), base + plan.off_refl, plan.output,
+                     plan.nblocks, per_xcd);
   PE_HIP(hipGetLastError(), "fused_blocks_kernel launch");
   return PE_OK;
 }
