@@ -1,16 +1,20 @@
-"""Benchmark: end-engine block compute on one MI355X.
+"""Benchmark: end-engine block compute on one MI355X (plus batched midsolve and serving).
 
-Workload (the largest one-GPU share of the end engine): a box of k^4 blocks (8x8x8x8
-nodes each) of the slice-24 section (3,3)(3,3)(3,3)(3,3), every open dimension computed.
-The box's child blocks (standard child section (4,3)(3,3)(3,3)(3,3): every block along the
-child dimension times the box cross-section) are resident in HBM as a dense slab filled with
+Headline workload (the largest one-GPU share of the end engine): a box of k^4 blocks
+(8x8x8x8 nodes each) of the slice-24 section (3,3)(3,3)(3,3)(3,3), mixed in the same
+launch with a box of k2^4 blocks (default 8^4) of the second slice-24 section
+(4,2)(2,4)(3,3)(3,3) (standard form (3,3)(2,4)(3,3)(4,2)), every open dimension computed.
+The child blocks the boxes' lines read (slice-25 child sections, every block along the
+child dimension times the box cross-section) are resident in HBM as dense slabs filled with
 seeded data (known ~ Bernoulli(0.9) bits, win ~ Bernoulli(0.5) & known).  One step = one
-pass of the end engine over the box: mode 0 (default) launches the fused block kernel once
-(every node gets all four dimensions and is written once); mode 1 launches four line passes,
-one per dimension, OR-merging into the box.  Inputs are on the device before timing starts.
+pass of the end engine over both boxes: mode 0 (default) launches the fused block kernel
+once (every node gets all its dimensions and is written once); mode 1 runs four line
+passes per box, OR-merging into the output.  Inputs are on the device before timing starts.
 
-Prints one JSON line.  Metric: superpositions/s = box output nodes per second (the
-reference's output nodes/second, pentago/mpi/toplevel.cc:83-94).
+Prints one JSON line.  Metric: superpositions/s = output nodes per second (the reference's
+output nodes/second, pentago/mpi/toplevel.cc:83-94).  The same line carries the batched
+midsolve (boards/s against plain per-board launches) and serving (p50/p99 latency at
+Poisson loads) measurements unless --no-mid / --no-serve.
 
 --copy-probe BYTES runs one plain device copy of BYTES bytes instead (a kernel of known
 traffic, used to scale the FETCH_SIZE / WRITE_SIZE counters).
@@ -28,6 +32,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 SECTION = ((3, 3), (3, 3), (3, 3), (3, 3))
+# standard form of the second section (4,2)(2,4)(3,3)(3,3), mixed into the same launch
+SECTION2 = ((3, 3), (2, 4), (3, 3), (4, 2))
 HBM_PEAK_GBPS = 8000.0
 
 
@@ -58,6 +64,77 @@ def build_box(P, hip, k, a0, mode=0):
     # every dimension's lines read the whole child slab once (the box is a cube)
     child_nodes = cshape[cd] * (8 * k) ** 3
     return out, kids, plan, child_nodes, child, n_child_blocks
+
+
+def child_slabs_for(P, counts, lo, hi):
+    """The child slabs (standard child section, block lo, block hi) that the lines through
+    the output box [lo, hi) read, and the algorithmic child nodes of those lines (each child
+    line counted once per line)."""
+    shape = P.section_shape(counts)
+    needs = []
+    child_nodes = 0
+    for d in range(4):
+        if sum(counts[d]) >= 9:
+            continue
+        info = P.line_describe(P.make_line(counts, d, tuple(lo[a] for a in range(4) if a != d)))
+        child = info.standard_child.counts
+        cshape = P.section_shape(child)
+        cd = info.child_dimension
+        perm = list(info.permutation)
+        clo = [0 if c == cd else lo[perm[c]] for c in range(4)]
+        chi = [(cshape[c] + 7) // 8 if c == cd else hi[perm[c]] for c in range(4)]
+        key = (child, tuple(clo), tuple(chi))
+        if key not in needs:
+            needs.append(key)
+        cross = 1
+        for a in range(4):
+            if a != d:
+                cross *= min(shape[a], 8 * hi[a]) - 8 * lo[a]
+        child_nodes += cshape[cd] * cross
+    return needs, child_nodes
+
+
+def boxes_bytes(P, specs):
+    total = 0
+    seen = set()
+    for counts, lo, hi in specs:
+        total += 64 * 4096 * int(np.prod([h - l for l, h in zip(lo, hi)]))
+        for key in child_slabs_for(P, counts, lo, hi)[0]:
+            if key not in seen:
+                seen.add(key)
+                total += 64 * 4096 * int(np.prod([h - l for l, h in zip(key[1], key[2])]))
+    return total
+
+
+def build_boxes(P, hip, specs, mode=0):
+    """Output slabs for each (section, lo, hi) box and the seeded child slabs they read, as
+    one plan.  Returns (outputs, children, plan, output nodes, child nodes)."""
+    outs, kids, kid_keys = [], [], []
+    out_nodes = child_nodes = 0
+    for counts, lo, hi in specs:
+        shape = P.section_shape(counts)
+        assert all(0 <= lo[a] < hi[a] and 8 * (hi[a] - 1) < shape[a] for a in range(4))
+        nblk = int(np.prod([h - l for l, h in zip(lo, hi)]))
+        buf = hip.Buffer(64 * 4096 * nblk)
+        buf.zero()
+        outs.append((counts, list(lo), list(hi), buf))
+        needs, cn = child_slabs_for(P, counts, lo, hi)
+        child_nodes += cn
+        out_nodes += int(np.prod([min(shape[a], 8 * hi[a]) - 8 * lo[a] for a in range(4)]))
+        for key in needs:
+            if key in kid_keys:
+                continue
+            child, clo, chi = key
+            nb = int(np.prod([h - l for l, h in zip(clo, chi)]))
+            kb = hip.Buffer(64 * 4096 * nb)
+            seed = 0x5EED0000 + sum(c * 10 ** i for i, c in enumerate(sum(child, ()))) + 7919 * len(kid_keys)
+            P.fill_seeded(kb.ptr, 4096 * nb, seed)
+            kid_keys.append(key)
+            kids.append((child, list(clo), list(chi), kb))
+    plan = P.Plan.slab([P.Slab.of(c, lo, hi, b.ptr) for c, lo, hi, b in outs],
+                       [P.Slab.of(c, lo, hi, b.ptr) for c, lo, hi, b in kids], mode)
+    assert plan.nodes == out_nodes
+    return outs, kids, plan, out_nodes, child_nodes
 
 
 def cpu_baseline(budget_s=15.0):
@@ -263,12 +340,15 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--k", type=int, default=0, help="box edge in blocks (0: largest fitting ~200 GB)")
+    ap.add_argument("--k2", type=int, default=8, help="edge of the second section's box (0: none)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", type=int, default=0, help="0: fused block kernel, 1: four line passes")
     ap.add_argument("--copy-probe", type=int, default=0, help="run one plain copy of this many bytes instead")
     ap.add_argument("--no-mid", action="store_true", help="skip the batched midsolve measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-end", action="store_true", help="skip the end-engine measurement (profiling the others)")
+    ap.add_argument("--mid-boards", type=int, default=4096, help="boards in the batched midsolve measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU; --gpus %d is not supported" % args.gpus,
@@ -281,101 +361,104 @@ def main():
     import pentago_amd as P
     from pentago_amd import hip
     P.init()
-    free, total = hip.mem_info()
-    shape = P.section_shape(SECTION)
-    k = args.k
-    if not k:
-        budget = min(200e9, 0.8 * free)
-        k = 1
-        while True:
-            kk = k + 1
-            need = 64 * 4096 * (kk ** 4 + 40 * kk ** 3)
-            if need > budget or 8 * kk > shape[0] - 8:
-                break
-            k = kk
-    a0 = 0
-    log("box k=%d (%d output blocks), free HBM %.1f GB" % (k, k ** 4, free / 1e9))
-    t0 = time.time()
-    out, kids, plan, child_nodes, child, n_child_blocks = build_box(P, hip, k, a0, args.mode)
-    hip.synchronize()
-    log("setup %.1f s: %d lines per pass, child slab %d blocks (%.1f GB), box %.1f GB" % (
-        time.time() - t0, k ** 3, n_child_blocks,
-        64 * 4096 * n_child_blocks / 1e9, 64 * 4096 * k ** 4 / 1e9))
+    result = {"metric": "superpositions/s", "value": None, "unit": "superpositions/s", "n_gpus": 1}
+    if not args.no_end:
+        free, total = hip.mem_info()
+        shape = P.section_shape(SECTION)
+        # the second section's box: k2^4 interior blocks, mixed into the same launch
+        k2, a2 = args.k2, 8
+        specs2 = [(SECTION2, [a2] * 4, [a2 + k2] * 4)] if k2 else []
+        k = args.k
+        if not k:
+            budget = min(200e9, 0.8 * free)
+            k = 1
+            while 8 * (k + 1) <= shape[0] - 8 and boxes_bytes(P, [(SECTION, [0] * 4, [k + 1] * 4)] + specs2) <= budget:
+                k += 1
+        specs = [(SECTION, [0] * 4, [k] * 4)] + specs2
+        log("boxes: %s, free HBM %.1f GB" % (specs, free / 1e9))
+        t0 = time.time()
+        outs, kids, plan, n_out, child_nodes = build_boxes(P, hip, specs, args.mode)
+        hip.synchronize()
+        log("setup %.1f s: %d output blocks (%.1f GB), %d child slabs (%.1f GB)" % (
+            time.time() - t0, sum(int(np.prod([h - l for l, h in zip(lo, hi)])) for _, lo, hi, _ in outs),
+            sum(b.nbytes for *_, b in outs) / 1e9, len(kids), sum(b.nbytes for *_, b in kids) / 1e9))
 
-    stream = hip.Stream()
-    n_out = (8 * k) ** 4
-    assert plan.nodes == n_out
-    for _ in range(args.warmup):
-        plan.run(stream.ptr)
-    stream.synchronize()
-    hip.synchronize()
-    ev0, ev1 = hip.Event(), hip.Event()
-    step_ms = []
-    t_start = time.time()
-    for step in range(args.steps):
-        ev0.record(stream)
-        plan.run(stream.ptr)
-        ev1.record(stream)
-        ev1.synchronize()
-        step_ms.append(ev0.elapsed_ms(ev1))
-        log("step %d: %.1f ms" % (step, step_ms[-1]))
-    stream.synchronize()
-    hip.synchronize()
-    wall = time.time() - t_start
-    ms_per_step = sum(step_ms) / args.steps
-    value = n_out / (ms_per_step / 1e3)
-    launches = 1 if args.mode == 0 else 4
-    avg_launch_ms = ms_per_step / launches
-    # algorithmic bytes per step: box output once + every dimension's child nodes once
-    alg_step = 64 * (n_out + 4 * child_nodes)
-    achieved = (alg_step / launches) / (avg_launch_ms / 1e3) / 1e9
-    kernel = "fused_blocks_kernel" if args.mode == 0 else "end_lines_kernel"
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "%s_box%d_traffic.json" % (kernel, k))
-    if os.path.exists(tfile):
-        with open(tfile) as f:
-            traffic = json.load(f).get("bytes_per_launch")
-    result = {
-        "metric": "superpositions/s",
-        "value": value,
-        "unit": "superpositions/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u64",
-        "data": "synthetic (seeded child blocks: known ~ Bernoulli(0.9), win ~ Bernoulli(0.5) & known)",
-        "config": {"workload": "end-engine box of %d^4 blocks of section 24-(3,3)^4, child slab of section 25-%s, "
-                               "all 4 dimensions" % (k, "".join("%d%d" % c for c in child)),
-                   "box_blocks": k ** 4, "output_nodes": n_out, "child_nodes_per_dimension": child_nodes, "mode": "fused" if args.mode == 0 else "line passes",
-                   "lines_per_pass": k ** 3},
-        "hbm_GBps_algorithmic": alg_step / (ms_per_step / 1e3) / 1e9,
-        "step_ms": step_ms,
-        "wall_s": wall,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
-                     "kernel": kernel, "bytes_per_launch": alg_step / launches,
-                     "avg_launch_ms": avg_launch_ms},
-    }
-    plan.close()
-    del out, kids
+        stream = hip.Stream()
+        for _ in range(args.warmup):
+            plan.run(stream.ptr)
+        stream.synchronize()
+        hip.synchronize()
+        ev0, ev1 = hip.Event(), hip.Event()
+        step_ms = []
+        t_start = time.time()
+        for step in range(args.steps):
+            ev0.record(stream)
+            plan.run(stream.ptr)
+            ev1.record(stream)
+            ev1.synchronize()
+            step_ms.append(ev0.elapsed_ms(ev1))
+            log("step %d: %.1f ms" % (step, step_ms[-1]))
+        stream.synchronize()
+        hip.synchronize()
+        wall = time.time() - t_start
+        ms_per_step = sum(step_ms) / args.steps
+        value = n_out / (ms_per_step / 1e3)
+        launches = 1 if args.mode == 0 else 4
+        avg_launch_ms = ms_per_step / launches
+        # algorithmic bytes per step: box output once + every line's child nodes once
+        alg_step = 64 * (n_out + child_nodes)
+        achieved = (alg_step / launches) / (avg_launch_ms / 1e3) / 1e9
+        kernel = "fused_blocks_kernel" if args.mode == 0 else "end_lines_kernel"
+        traffic = None
+        tag = "box%d" % k + ("_mixed%d" % k2 if k2 else "")
+        tfile = os.path.join(ROOT, "profiles", "%s_%s_traffic.json" % (kernel, tag))
+        if os.path.exists(tfile):
+            with open(tfile) as f:
+                traffic = json.load(f).get("bytes_per_launch")
+        result = {
+            "metric": "superpositions/s",
+            "value": value,
+            "unit": "superpositions/s",
+            "n_gpus": 1,
+            "steps": args.steps,
+            "warmup": args.warmup,
+            "ms_per_step": ms_per_step,
+            "higher_is_better": True,
+            "scaling": "weak",
+            "vs_baseline": None,
+            "dtype": "u64",
+            "data": "synthetic (seeded child blocks: known ~ Bernoulli(0.9), win ~ Bernoulli(0.5) & known)",
+            "config": {"workload": "end-engine: box of %d^4 blocks of section 24-(3,3)^4%s, all open dimensions, "
+                                   "child slabs of the slice-25 children" % (
+                                       k, (" mixed with a box of %d^4 blocks of section 24-%s" % (
+                                           k2, "".join("%d%d" % c for c in SECTION2))) if k2 else ""),
+                       "boxes": [{"section": "".join("%d%d" % c for c in sec), "lo": lo, "hi": hi} for sec, lo, hi in specs],
+                       "output_nodes": n_out, "child_nodes_read": child_nodes,
+                       "mode": "fused" if args.mode == 0 else "line passes"},
+            "hbm_GBps_algorithmic": alg_step / (ms_per_step / 1e3) / 1e9,
+            "step_ms": step_ms,
+            "wall_s": wall,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBPS, "traffic": traffic,
+                         "kernel": kernel, "bytes_per_launch": alg_step / launches,
+                         "avg_launch_ms": avg_launch_ms},
+        }
+        plan.close()
+        del outs, kids
     if not args.no_mid:
-        result["midsolve"] = mid_bench(P, hip)
+        result["midsolve"] = mid_bench(P, hip, args.mid_boards)
     if not args.no_serve:
         result["serving"] = serve_bench(P)
-    if not args.no_cpu:
+    if not args.no_cpu and not args.no_end:
         try:
             result["cpu_baseline"] = cpu_baseline(args.cpu_budget)
         except Exception as e:   # the baseline is reported, never required
             result["cpu_baseline"] = {"error": str(e)}
-        if not args.no_mid:
-            try:
-                result["midsolve"]["cpu_baseline"] = mid_cpu_baseline()
-            except Exception as e:
-                result["midsolve"]["cpu_baseline"] = {"error": str(e)}
+    if not args.no_cpu and not args.no_mid:
+        try:
+            result["midsolve"]["cpu_baseline"] = mid_cpu_baseline()
+        except Exception as e:
+            result["midsolve"]["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(result), flush=True)
 
 

@@ -137,6 +137,10 @@ typedef struct {
  * The output section must have at most 34 stones. */
 pe_status pe_end_slab_plan_create(const pe_slab* output, int nchildren, const pe_slab* children, int mode,
                                   pe_end_plan** plan);
+/* The same for several output slabs (of one or several sections) in one plan: mode 0 runs
+ * every block of every output in one fused launch. */
+pe_status pe_end_slabs_plan_create(int noutputs, const pe_slab* outputs, int nchildren, const pe_slab* children,
+                                   int mode, pe_end_plan** plan);
 
 /* Synchronous host-buffer convenience: copies the input to the device, computes, copies the
  * output back.  Same layouts as above. */

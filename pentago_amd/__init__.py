@@ -147,6 +147,7 @@ def lib():
             "pe_end_plan_create": (C.c_int, [C.c_int, P(LineJob), P(V)]),
             "pe_end_plan_run": (C.c_int, [V, V]),
             "pe_end_slab_plan_create": (C.c_int, [P(Slab), C.c_int, P(Slab), C.c_int, P(V)]),
+            "pe_end_slabs_plan_create": (C.c_int, [C.c_int, P(Slab), C.c_int, P(Slab), C.c_int, P(V)]),
             "pe_end_plan_nodes": (C.c_int64, [V]),
             "pe_end_plan_destroy": (C.c_int, [V]),
             "pe_fill_seeded": (C.c_int, [V, C.c_int64, C.c_uint64, C.c_int64, V]),
@@ -260,13 +261,15 @@ class Plan:
 
     @classmethod
     def slab(cls, output, children, mode=0):
-        """Plan computing every block of the output slab from child slabs (mode 0 fused,
-        mode 1 four line passes)."""
+        """Plan computing every block of the output slab (or list of output slabs) from
+        child slabs (mode 0 fused, mode 1 four line passes)."""
         self = cls.__new__(cls)
+        outs = list(output) if isinstance(output, (list, tuple)) else [output]
         n = len(children)
         arr = (Slab * max(n, 1))(*children)
+        oarr = (Slab * len(outs))(*outs)
         h = C.c_void_p()
-        _check(lib().pe_end_slab_plan_create(C.byref(output), n, arr, mode, C.byref(h)), "pe_end_slab_plan_create")
+        _check(lib().pe_end_slabs_plan_create(len(outs), oarr, n, arr, mode, C.byref(h)), "pe_end_slabs_plan_create")
         self.handle = h
         self.nodes = lib().pe_end_plan_nodes(h)
         return self

@@ -18,7 +18,8 @@ namespace pe {
 pe_status plan_create(const std::vector<LineHost>& lines, EndPlan* plan);
 pe_status plan_run(const EndPlan& plan, hipStream_t stream);
 void plan_destroy(EndPlan* plan);
-pe_status fused_plan_create(const pe_slab& out, int nch, const pe_slab* ch, FusedPlan* plan, std::string* err);
+pe_status fused_plan_create(int nout, const pe_slab* outs, int nch, const pe_slab* ch, FusedPlan* plan,
+                            std::string* err);
 pe_status fused_plan_run(const FusedPlan& plan, hipStream_t stream);
 void fused_plan_destroy(FusedPlan* plan);
 pe_status slab_line_passes(const pe_slab& out, int nch, const pe_slab* ch, std::vector<EndPlan>* passes,
@@ -225,18 +226,30 @@ pe_status pe_end_plan_create(int n, const pe_line_job* jobs, pe_end_plan** out) 
 
 pe_status pe_end_slab_plan_create(const pe_slab* output, int nchildren, const pe_slab* children, int mode,
                                   pe_end_plan** out) {
-  if (!output || !out || nchildren < 0 || (nchildren && !children) || !output->data)
+  return pe_end_slabs_plan_create(1, output, nchildren, children, mode, out);
+}
+
+pe_status pe_end_slabs_plan_create(int noutputs, const pe_slab* outputs, int nchildren, const pe_slab* children,
+                                   int mode, pe_end_plan** out) {
+  if (noutputs <= 0 || !outputs || !out || nchildren < 0 || (nchildren && !children))
     return fail(PE_ERR_INVALID, "bad slab plan arguments");
+  for (int k = 0; k < noutputs; k++)
+    if (!outputs[k].data) return fail(PE_ERR_INVALID, "output slab without memory");
   if (mode != 0 && mode != 1) return fail(PE_ERR_INVALID, "unknown slab plan mode");
   auto* plan = new pe_end_plan();
   std::string err;
-  pe_status s;
+  pe_status s = PE_OK;
   if (mode == 0) {
     plan->is_fused = true;
-    s = fused_plan_create(*output, nchildren, children, &plan->fused, &err);
+    s = fused_plan_create(noutputs, outputs, nchildren, children, &plan->fused, &err);
     plan->nodes = plan->fused.output_nodes;
   } else {
-    s = slab_line_passes(*output, nchildren, children, &plan->passes, &plan->nodes, &err);
+    plan->nodes = 0;
+    for (int k = 0; k < noutputs && s == PE_OK; k++) {
+      int64_t nodes = 0;
+      s = slab_line_passes(outputs[k], nchildren, children, &plan->passes, &nodes, &err);
+      plan->nodes += nodes;
+    }
   }
   if (s != PE_OK) {
     for (auto& p : plan->passes) plan_destroy(&p);

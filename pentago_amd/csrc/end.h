@@ -97,14 +97,13 @@ struct BlockDev {
   int32_t sec;
   int32_t b[4];
   int32_t line[4];          // -1 if the dimension is closed
-  int64_t out;              // node offset of the block in the output slab
+  uint64_t* out;            // the block's 4096-node slot in its output slab
 };
 
 struct FusedPlan {
   void* blob = nullptr;
   int nblocks = 0;
   size_t off_lines = 0, off_secs = 0, off_fields = 0, off_moves = 0, off_refl = 0;
-  uint64_t* output = nullptr;
   int64_t output_nodes = 0;
 };
 

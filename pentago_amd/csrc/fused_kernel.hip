@@ -95,7 +95,7 @@ __global__ __launch_bounds__(kThreads)
 void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __restrict__ lines,
                          const SecDev* __restrict__ secs, const uint64_t* __restrict__ fields,
                          const uint16_t* __restrict__ movetab, const uint8_t* __restrict__ refl,
-                         uint64_t* __restrict__ output, int nblocks, int blocks_per_xcd) {
+                         int nblocks, int blocks_per_xcd) {
   // Workgroups are dispatched to the 8 XCDs round-robin: workgroup i runs on XCD i % 8.
   // XCD x takes the contiguous run of blocks [x * blocks_per_xcd, ...), so the blocks that
   // share child lines (neighbours along the last axes) and the two halves of a 128-byte line
@@ -211,7 +211,7 @@ void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __r
     o1.w[k] = (r0.w[k] & ~imm) | w[0];
     PE_FENCE();
   }
-  uint64_t* o = output + 8 * (B.out + p);
+  uint64_t* o = B.out + 8 * p;
   st(o, o0);
   st(o + 4, o1);
 }
@@ -277,52 +277,69 @@ static pe_status slab_lines(const pe_slab& out, int nch, const pe_slab* ch, std:
   return PE_OK;
 }
 
-pe_status fused_plan_create(const pe_slab& out, int nch, const pe_slab* ch, FusedPlan* plan, std::string* err) {
+pe_status fused_plan_create(int nout, const pe_slab* outs, int nch, const pe_slab* ch, FusedPlan* plan,
+                            std::string* err) {
   DevTables T;
   if (device_tables(&T, err) != PE_OK) return PE_ERR_DEVICE;
-  std::vector<LineHost> lines;
-  std::map<std::tuple<int, int, int, int>, int> index;
-  pe_status st = slab_lines(out, nch, ch, &lines, &index, err);
-  if (st != PE_OK) return st;
-  Section s;
-  std::memcpy(s.c, out.section.counts, 8);
-  SecDev sec;
-  std::memset(&sec, 0, sizeof sec);
-  sec.turn = s.sum() & 1;
-  std::vector<uint64_t> fields;
-  section_field_tables(s, &fields, sec.field_off);
-  std::vector<uint16_t> movetab;
-  st = section_move_tables(s, &movetab, sec.move_off, sec.nmoves, err);
-  if (st != PE_OK) return st;
   const auto& Tb = tables();
-  for (int a = 0; a < 4; a++) {
-    const int b = rmin_bucket(s.c[a][0], s.c[a][1]);
-    sec.rmin_off[a] = Tb.rmin_offsets[b];
-    sec.rmin_size[a] = Tb.rmin_offsets[b + 1] - Tb.rmin_offsets[b];
-  }
+  std::vector<LineHost> lines;
+  std::vector<SecDev> secs;
+  std::vector<uint64_t> fields;
+  std::vector<uint16_t> movetab;
   std::vector<BlockDev> blocks;
   int64_t nodes = 0;
-  int b[4];
-  for (b[0] = out.lo[0]; b[0] < out.hi[0]; b[0]++)
-    for (b[1] = out.lo[1]; b[1] < out.hi[1]; b[1]++)
-      for (b[2] = out.lo[2]; b[2] < out.hi[2]; b[2]++)
-        for (b[3] = out.lo[3]; b[3] < out.hi[3]; b[3]++) {
-          BlockDev B;
-          B.sec = 0;
-          int n = 1;
-          for (int a = 0; a < 4; a++) { B.b[a] = b[a]; n *= std::min(8, sec.rmin_size[a] - 8 * b[a]); }
-          for (int d = 0; d < 4; d++) {
-            B.line[d] = -1;
-            if (s.c[d][0] + s.c[d][1] >= 9) continue;
-            int o[3], k = 0;
-            for (int a = 0; a < 4; a++) if (a != d) o[k++] = b[a];
-            B.line[d] = index.at(std::make_tuple(d, o[0], o[1], o[2]));
+  for (int k = 0; k < nout; k++) {
+    const pe_slab& out = outs[k];
+    if (!out.data) { *err = "output slab without memory"; return PE_ERR_INVALID; }
+    std::vector<LineHost> sl;
+    std::map<std::tuple<int, int, int, int>, int> index;
+    pe_status st = slab_lines(out, nch, ch, &sl, &index, err);
+    if (st != PE_OK) return st;
+    Section s;
+    std::memcpy(s.c, out.section.counts, 8);
+    SecDev sec;
+    std::memset(&sec, 0, sizeof sec);
+    sec.turn = s.sum() & 1;
+    std::vector<uint64_t> f;
+    section_field_tables(s, &f, sec.field_off);
+    std::vector<uint16_t> mt;
+    st = section_move_tables(s, &mt, sec.move_off, sec.nmoves, err);
+    if (st != PE_OK) return st;
+    for (int a = 0; a < 4; a++) {
+      sec.field_off[a] += int32_t(fields.size() / 4);
+      if (sec.move_off[a] >= 0) sec.move_off[a] += int32_t(movetab.size());
+      const int b = rmin_bucket(s.c[a][0], s.c[a][1]);
+      sec.rmin_off[a] = Tb.rmin_offsets[b];
+      sec.rmin_size[a] = Tb.rmin_offsets[b + 1] - Tb.rmin_offsets[b];
+    }
+    fields.insert(fields.end(), f.begin(), f.end());
+    movetab.insert(movetab.end(), mt.begin(), mt.end());
+    const int line0 = int(lines.size());
+    for (auto& h : sl) lines.push_back(std::move(h));
+    int b[4];
+    for (b[0] = out.lo[0]; b[0] < out.hi[0]; b[0]++)
+      for (b[1] = out.lo[1]; b[1] < out.hi[1]; b[1]++)
+        for (b[2] = out.lo[2]; b[2] < out.hi[2]; b[2]++)
+          for (b[3] = out.lo[3]; b[3] < out.hi[3]; b[3]++) {
+            BlockDev B;
+            B.sec = int32_t(secs.size());
+            int n = 1;
+            for (int a = 0; a < 4; a++) { B.b[a] = b[a]; n *= std::min(8, sec.rmin_size[a] - 8 * b[a]); }
+            for (int d = 0; d < 4; d++) {
+              B.line[d] = -1;
+              if (s.c[d][0] + s.c[d][1] >= 9) continue;
+              int o[3], kk = 0;
+              for (int a = 0; a < 4; a++) if (a != d) o[kk++] = b[a];
+              B.line[d] = line0 + index.at(std::make_tuple(d, o[0], o[1], o[2]));
+            }
+            B.out = static_cast<uint64_t*>(out.data) + 8 * 4096 * slab_index(out, b);
+            blocks.push_back(B);
+            nodes += n;
           }
-          B.out = 4096 * slab_index(out, b);
-          blocks.push_back(B);
-          nodes += n;
-        }
-  // blob: blocks | lines | section | fields | offsets | symmetries
+    secs.push_back(sec);
+  }
+  if (blocks.size() > size_t(0x7fffffff) / kParts) { *err = "fused plan too large"; return PE_ERR_CAPACITY; }
+  // blob: blocks | lines | sections | fields | moves | offsets | symmetries
   std::vector<LineDev> devs(lines.size());
   std::vector<int64_t> offs;
   std::vector<uint8_t> refl;
@@ -347,7 +364,7 @@ pe_status fused_plan_create(const pe_slab& out, int nch, const pe_slab* ch, Fuse
   auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
   const size_t o_lines = up(sizeof(BlockDev) * blocks.size());
   const size_t o_secs = o_lines + up(sizeof(LineDev) * devs.size());
-  const size_t o_fields = o_secs + up(sizeof(SecDev));
+  const size_t o_fields = o_secs + up(sizeof(SecDev) * secs.size());
   const size_t o_moves = o_fields + up(sizeof(uint64_t) * fields.size());
   const size_t o_offs = o_moves + up(sizeof(uint16_t) * std::max<size_t>(movetab.size(), 1));
   const size_t o_refl = o_offs + up(sizeof(int64_t) * offs.size());
@@ -359,7 +376,7 @@ pe_status fused_plan_create(const pe_slab& out, int nch, const pe_slab* ch, Fuse
   std::vector<uint8_t> blob(bytes, 0);
   std::memcpy(blob.data(), blocks.data(), sizeof(BlockDev) * blocks.size());
   std::memcpy(blob.data() + o_lines, devs.data(), sizeof(LineDev) * devs.size());
-  std::memcpy(blob.data() + o_secs, &sec, sizeof sec);
+  std::memcpy(blob.data() + o_secs, secs.data(), sizeof(SecDev) * secs.size());
   std::memcpy(blob.data() + o_fields, fields.data(), sizeof(uint64_t) * fields.size());
   if (!movetab.empty()) std::memcpy(blob.data() + o_moves, movetab.data(), sizeof(uint16_t) * movetab.size());
   std::memcpy(blob.data() + o_offs, offs.data(), sizeof(int64_t) * offs.size());
@@ -376,7 +393,6 @@ pe_status fused_plan_create(const pe_slab& out, int nch, const pe_slab* ch, Fuse
   plan->off_fields = o_fields;
   plan->off_moves = o_moves;
   plan->off_refl = o_refl;
-  plan->output = static_cast<uint64_t*>(out.data);
   plan->output_nodes = nodes;
   return PE_OK;
 }
@@ -394,8 +410,8 @@ pe_status fused_plan_run(const FusedPlan& plan, hipStream_t stream) {
                      reinterpret_cast<const BlockDev*>(base), reinterpret_cast<const LineDev*>(base + plan.off_lines),
                      reinterpret_cast<const SecDev*>(base + plan.off_secs),
                      reinterpret_cast<const uint64_t*>(base + plan.off_fields),
-                     reinterpret_cast<const uint16_t*>(base + plan.off_moves), base + plan.off_refl, plan.output,
-                     plan.nblocks, per_xcd);
+                     reinterpret_cast<const uint16_t*>(base + plan.off_moves), base + plan.off_refl, plan.nblocks,
+                     per_xcd);
   PE_HIP(hipGetLastError(), "fused_blocks_kernel launch");
   return PE_OK;
 }

@@ -142,3 +142,33 @@ def test_slab_plan_rejects_missing_children():
     out = hip.Buffer(64 * 4096)
     with pytest.raises(P.PentagoError):
         P.Plan.slab(P.Slab.of(bench.SECTION, [0] * 4, [1] * 4, out.ptr), [], 0)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_mixed_sections_in_one_plan(mode):
+    """Boxes of two sections (the benchmark's mix, at 1^4 and 2^4 blocks) in one plan: every
+    block of both against the oracle."""
+    P.init()
+    specs = [(bench.SECTION, [5] * 4, [6] * 4), (bench.SECTION2, [9, 9, 9, 9], [10, 11, 10, 11])]
+    outs, kids, plan, n_out, child_nodes = bench.build_boxes(P, hip, specs, mode)
+    plan.run()
+    hip.synchronize()
+    S = SlabSet()
+    for child, lo, hi, buf in kids:
+        nb = int(np.prod([h - l for l, h in zip(lo, hi)]))
+        S.add(child, lo, hi, buf.download(np.zeros(8 * 4096 * nb, dtype=np.uint64)).reshape(nb, 4096, 8))
+    for counts, lo, hi, buf in outs:
+        nb = int(np.prod([h - l for l, h in zip(lo, hi)]))
+        got = buf.download(np.zeros(8 * 4096 * nb, dtype=np.uint64)).reshape(nb, 4096, 8)
+        shape = O.section_shape(counts)
+        idx = 0
+        for b0 in range(lo[0], hi[0]):
+            for b1 in range(lo[1], hi[1]):
+                for b2 in range(lo[2], hi[2]):
+                    for b3 in range(lo[3], hi[3]):
+                        blk = (b0, b1, b2, b3)
+                        print("mode", mode, "section", counts, "block", blk, flush=True)
+                        expect = oracle_block(counts, blk, S.block)
+                        assert np.array_equal(got[idx].reshape(-1)[:expect.size], expect), (counts, blk)
+                        idx += 1
+    plan.close()
