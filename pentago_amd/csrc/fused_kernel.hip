@@ -61,9 +61,14 @@ __device__ __forceinline__ void st(uint64_t* p, S4 v) {
 enum { FH = 0, FV = 1, FLO = 2, FHI = 3, FAS = 4 };
 
 // Field words of one node: word k of field f of table t (0: F0, 1: F1, 2: Fm), quadrant q.
+// Quadrants 0 and 1 are read from global memory (the same entry for the whole wave),
+// quadrants 2 and 3 from the block's copy of its eight entries in LDS.
 struct NodeFields {
-  const uint64_t* base[4];
-  __device__ __forceinline__ uint64_t w(int q, int k, int t, int f) const { return base[q][15 * k + 5 * t + f]; }
+  const uint64_t* glob[2];
+  const uint64_t* lds[2];
+  __device__ __forceinline__ uint64_t w(int q, int k, int t, int f) const {
+    return q < 2 ? glob[q][15 * k + 5 * t + f] : lds[q - 2][15 * k + 5 * t + f];
+  }
 };
 
 // Word k of a pattern-group term for the three quantities: x[0] mover's side, x[1] other
@@ -113,6 +118,14 @@ void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __r
   const int lane = threadIdx.x & 63;
   const int bs0 = min(8, S.rmin_size[0] - 8 * B.b[0]), bs1 = min(8, S.rmin_size[1] - 8 * B.b[1]);
   const int bs2 = min(8, S.rmin_size[2] - 8 * B.b[2]), bs3 = min(8, S.rmin_size[3] - 8 * B.b[3]);
+  // the block's field entries of quadrants 2 and 3 (eight each) staged in LDS
+  __shared__ uint64_t sf[2][8][60];
+  for (int i = threadIdx.x; i < 2 * 8 * 60; i += kThreads) {
+    const int q = 2 + i / 480, r = (i / 60) % 8, w = i % 60;
+    const int bsq = q == 2 ? bs2 : bs3;
+    sf[q - 2][r][w] = r < bsq ? fields[4 * int64_t(S.field_off[q]) + 60 * int64_t(8 * B.b[q] + r) + w] : 0;
+  }
+  __syncthreads();
   int loc[4] = {wave >> 3, wave & 7, lane >> 3, lane & 7};
   if (loc[0] >= bs0 || loc[1] >= bs1 || loc[2] >= bs2 || loc[3] >= bs3) return;
   const int p = ((loc[0] * bs1 + loc[1]) * bs2 + loc[2]) * bs3 + loc[3];
@@ -176,8 +189,10 @@ void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __r
   //      inside a word, and the three groups reduced over r3 become ORs across words ----
   NodeFields F;
 #pragma unroll
-  for (int a = 0; a < 4; a++)
-    F.base[a] = fields + 4 * int64_t(S.field_off[a]) + 60 * int64_t(8 * B.b[a] + loc[a]);
+  for (int a = 0; a < 2; a++)
+    F.glob[a] = fields + 4 * int64_t(S.field_off[a]) + 60 * int64_t(8 * B.b[a] + loc[a]);
+  F.lds[0] = sf[0][loc[2]];
+  F.lds[1] = sf[1][loc[3]];
   uint64_t A[3] = {0, 0, 0}, Bv[3] = {0, 0, 0}, Cv[3] = {0, 0, 0};
 #pragma unroll
   for (int k = 0; k < 4; k++) {
