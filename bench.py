@@ -181,6 +181,37 @@ def copy_probe(nbytes):
     print(json.dumps({"copy_probe_bytes": 4 * n, "read": 4 * n, "written": 4 * n}), flush=True)
 
 
+SECTION30 = ((3, 4), (4, 4), (4, 4), (4, 3))   # standard form of (4,4)(4,3)(3,4)(4,4)
+
+
+def section30_bench(P, hip, steps):
+    """The slice-30 section (4,4)(4,3)(3,4)(4,4) (standard form (3,4)(4,4)(4,4)(4,3), shape
+    318 x 160 x 160 x 318): a box of 10 x 20 x 20 x 10 blocks (all blocks along the two
+    middle axes; its dimension-2 children come through a reflection) in one fused launch."""
+    specs = [(SECTION30, [0, 0, 0, 0], [10, 20, 20, 10])]
+    outs, kids, plan, n_out, child_nodes = build_boxes(P, hip, specs, 0)
+    stream = hip.Stream()
+    plan.run(stream.ptr)
+    stream.synchronize()
+    ev0, ev1 = hip.Event(), hip.Event()
+    ms = []
+    for _ in range(max(steps, 1)):
+        ev0.record(stream)
+        plan.run(stream.ptr)
+        ev1.record(stream)
+        ev1.synchronize()
+        ms.append(ev0.elapsed_ms(ev1))
+    plan.close()
+    t = sum(ms) / len(ms)
+    alg = 64 * (n_out + child_nodes)
+    out = {"workload": "box [0,10)x[0,20)x[0,20)x[0,10) of section 30-%s" % "".join("%d%d" % c for c in SECTION30),
+           "output_nodes": n_out, "child_nodes_read": child_nodes, "ms": t,
+           "superpositions_per_s": n_out / (t / 1e3), "algorithmic_GBps": alg / (t / 1e3) / 1e9,
+           "roofline_frac": alg / (t / 1e3) / 1e9 / HBM_PEAK_GBPS}
+    log("section 30 box: %s" % out)
+    return out
+
+
 def mid_bench(P, hip, n_boards=4096, seed=2):
     """Batched midsolve: n seeded boards with 18..30 stones in one work-queue launch
     (autotuned tile and workgroups) against one plain pe_midsolve_internal call per board;
@@ -349,6 +380,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-end", action="store_true", help="skip the end-engine measurement (profiling the others)")
     ap.add_argument("--mid-boards", type=int, default=4096, help="boards in the batched midsolve measurement")
+    ap.add_argument("--no-section30", dest="section30", action="store_false",
+                    help="skip the slice-30 section box measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU; --gpus %d is not supported" % args.gpus,
@@ -445,6 +478,8 @@ def main():
         }
         plan.close()
         del outs, kids
+    if not args.no_end and args.section30:
+        result["section30"] = section30_bench(P, hip, args.steps)
     if not args.no_mid:
         result["midsolve"] = mid_bench(P, hip, args.mid_boards)
     if not args.no_serve:

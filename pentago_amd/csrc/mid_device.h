@@ -111,9 +111,10 @@ __device__ inline void mid_prep_entry(const MidBoardDev& B, int64_t idx, const D
     out.after[0] = after.w[0]; out.after[1] = after.w[1];
     out.before[0] = before.w[0]; out.before[1] = before.w[1];
     B.wins1[e] = out;
-  } else if (idx >= n_set0 + n_wins1 + I.cs1ps_off[spots + 1]) {
-    // subset s1p of each level as a bit mask (the unranking of subsets.h:33-48, done once)
-    const int64_t e = idx - n_set0 - n_wins1 - I.cs1ps_off[spots + 1];
+  } else {
+    // one thread per (level, s1p): the subset s1p as a bit mask, then its cs1ps row
+    // (internal.h:133-148: remap s1p when a stone of the mover lands at empty1[j])
+    const int64_t e = idx - n_set0 - n_wins1;
     const int n = level_of(I.sets1p_off, spots, e);
     const int s1p = int(e - I.sets1p_off[n]);
     const int k0 = n >> 1, k1 = n - k0;
@@ -121,25 +122,18 @@ __device__ inline void mid_prep_entry(const MidBoardDev& B, int64_t idx, const D
     uint32_t bits = 0;
     for (int a = 0; a < k1; a++) bits |= 1u << (set1p >> (5 * a) & 0x1f);
     B.set1p_bits[e] = bits;
-  } else {
-    // cs1ps (internal.h:133-148): remap s1p when a stone of the mover lands at empty1[j]
-    const int64_t e = idx - n_set0 - n_wins1;
-    const int n = level_of(I.cs1ps_off, spots, e);
-    const int index = int(e - I.cs1ps_off[n]);
-    const int k0 = n >> 1, k1 = n - k0;
-    const int s1p = index / (spots - k0);
-    const int j = index - s1p * (spots - k0);
-    const uint64_t set1p = dsubset(C, spots - k0, k1, s1p);
-    uint16_t c = uint16_t(s1p);
-    for (int a = 0; a < k1; a++) {
-      const int v = int(set1p >> (5 * a) & 0x1f);
-      if (j < v) c = uint16_t(c - dchoose(C, v - 1, a));
+    uint16_t* row = B.cs1ps + I.cs1ps_off[n] + int64_t(s1p) * (spots - k0);
+    for (int j = 0; j < spots - k0; j++) {
+      uint16_t c = uint16_t(s1p);
+      for (int a = 0; a < k1; a++) {
+        const int v = int(set1p >> (5 * a) & 0x1f);
+        if (j < v) c = uint16_t(c - dchoose(C, v - 1, a));
+      }
+      row[j] = c;
     }
-    B.cs1ps[e] = c;
   }
 }
 
-// One position (internal.h:358-408), both aggressive passes at once.
 // A level holds fewer than 2^31 positions (at most 17,153,136 for 18 empty cells).
 __device__ inline void mid_position(const MidBoardDev& B, int n, int64_t pos64, uint64_t* __restrict__ work,
                                     const DevTables& T) {

@@ -130,7 +130,7 @@ pe_status mid_solve_levels(const std::vector<MidInfo>& infos, uint8_t* arena, st
   std::vector<int64_t> prefix(size_t(nb + 1) * (steps + 1), 0);
   for (int b = 0; b < nb; b++) {
     const MidSizes s = mid_sizes(infos[b]);
-    prefix[b + 1] = prefix[b] + s.set0 + s.wins1 + s.cs1ps + s.set1p;
+    prefix[b + 1] = prefix[b] + s.set0 + s.wins1 + s.set1p;
   }
   for (int t = 0; t < steps; t++) {
     int64_t* p = prefix.data() + size_t(nb + 1) * (t + 1);

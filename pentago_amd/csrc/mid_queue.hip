@@ -146,7 +146,7 @@ pe_status mid_plan_create(const std::vector<MidInfo>& infos, const MidConfig& cf
     B.results = res + 19 * size_t(L.small[k]);
     std::memcpy(&sdev[k], &B, sizeof B);
     const MidSizes s = mid_sizes(B.I);
-    prefix[k + 1] = prefix[k] + s.set0 + s.wins1 + s.cs1ps + s.set1p;
+    prefix[k + 1] = prefix[k] + s.set0 + s.wins1 + s.set1p;
   }
   plan->small_prep_total = prefix[L.small.size()];
   plan->big_prep_total.resize(L.big.size());
@@ -162,7 +162,7 @@ pe_status mid_plan_create(const std::vector<MidInfo>& infos, const MidConfig& cf
     const MidSizes s = mid_sizes(B.I);
     int64_t* p = prefix + L.small.size() + 1 + 2 * k;
     p[0] = 0;
-    p[1] = s.set0 + s.wins1 + s.cs1ps + s.set1p;
+    p[1] = s.set0 + s.wins1 + s.set1p;
     plan->big_prep_total[k] = p[1];
   }
   if (alloc_stream) {

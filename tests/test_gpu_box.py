@@ -172,3 +172,50 @@ def test_mixed_sections_in_one_plan(mode):
                         assert np.array_equal(got[idx].reshape(-1)[:expect.size], expect), (counts, blk)
                         idx += 1
     plan.close()
+
+
+class DeviceSlabs:
+    """Child blocks read from device slabs on demand (for boxes too large to copy to the host)."""
+
+    def __init__(self, kids):
+        self.kids = kids
+
+    def block(self, counts, cb):
+        for c, lo, hi, buf in self.kids:
+            if c != counts or any(not (lo[a] <= cb[a] < hi[a]) for a in range(4)):
+                continue
+            idx = 0
+            for a in range(4):
+                idx = idx * (hi[a] - lo[a]) + (cb[a] - lo[a])
+            shape = O.section_shape(counts)
+            bshape = [min(shape[a], 8 * (cb[a] + 1)) - 8 * cb[a] for a in range(4)]
+            n = int(np.prod(bshape))
+            return buf.download(np.zeros(8 * n, dtype=np.uint64), offset=64 * 4096 * idx)
+        raise KeyError((counts, cb))
+
+
+def test_full_size_mixed_box_samples():
+    """The benchmark's full workload (22^4 blocks of (3,3)^4 and 8^4 blocks of the second
+    section, ~193 GB resident): two sampled blocks of each section against the oracle."""
+    P.init()
+    free, _ = hip.mem_info()
+    specs = [(bench.SECTION, [0] * 4, [22] * 4), (bench.SECTION2, [8] * 4, [16] * 4)]
+    need = bench.boxes_bytes(P, specs)
+    if need > 0.9 * free:
+        pytest.skip("needs %.0f GB of device memory, %.0f GB free" % (need / 1e9, free / 1e9))
+    outs, kids, plan, n_out, child_nodes = bench.build_boxes(P, hip, specs, 0)
+    plan.run()
+    hip.synchronize()
+    D = DeviceSlabs(kids)
+    rng = np.random.default_rng(2024)
+    for counts, lo, hi, buf in outs:
+        for _ in range(2):
+            blk = tuple(int(rng.integers(lo[a], hi[a])) for a in range(4))
+            idx = 0
+            for a in range(4):
+                idx = idx * (hi[a] - lo[a]) + (blk[a] - lo[a])
+            expect = oracle_block(counts, blk, D.block)
+            got = buf.download(np.zeros(expect.size, dtype=np.uint64), offset=64 * 4096 * idx)
+            print("full box", counts, "block", blk, flush=True)
+            assert np.array_equal(got, expect), (counts, blk)
+    plan.close()
