@@ -78,3 +78,22 @@ def test_poll_and_rejections():
         time.sleep(0.0005)
     server.close()
     assert out is not None and out[0] == P.midsolve_internal(b)
+
+
+def test_http_front_end_known_answer():
+    """GET /<board> through the HTTP front end: the 163 known values of
+    pentago/mid/benchmark_test.cc:39-58 as a JSON object of board names."""
+    import json
+    import urllib.request
+    from pentago_amd import http_server as H
+    from tests.common import golden
+    kat = golden("mid_kat.json")
+    httpd, solver = H.serve(port=0)
+    try:
+        port = httpd.server_address[1]
+        with urllib.request.urlopen("http://127.0.0.1:%d/%d" % (port, kat["board"]), timeout=60) as r:
+            got = json.loads(r.read().decode())
+    finally:
+        httpd.shutdown()
+        solver.close()
+    assert got == {"%d%s" % (b, "m" if m else ""): v for b, m, v in kat["values"]}
