@@ -152,6 +152,48 @@ pe_status pe_end_compute_line_host(const pe_line* line, const void* h_input, voi
  * (seed, node index, word). */
 pe_status pe_fill_seeded(void* d_nodes, int64_t nodes, uint64_t seed, int64_t first_node, void* stream);
 
+/* ---------------- end engine: counts, samples, slices ---------------- */
+
+/* Win counts of every block of a slab: count_block_wins (pentago/end/block_store.cc:330-349),
+ * i.e. the sum over the block's nodes of popcounts_over_stabilizers
+ * (pentago/base/count.cc:254-262).  d_counts (device) receives 4 u64 per block, C-order over
+ * the box: wins, wins-or-ties (for the player to move), total distinct positions, and the
+ * number of nodes whose popcounts are not multiples of their stabilizer size (the reference
+ * asserts zero).  Asynchronous on `stream`. */
+pe_status pe_end_count_blocks(const pe_slab* slab, uint64_t* d_counts, void* stream);
+/* The same summed per slab, synchronous: counts[k] = totals over slab k's blocks. */
+pe_status pe_end_count_slabs(int n, const pe_slab* slabs, uint64_t (*counts)[4]);
+
+/* Sparse samples: the node of each board (a rotation-standardized board of the slab's
+ * section inside the slab's box; quadrant q in bits 16q..16q+15, radix 3) copied to
+ * h_nodes (64 bytes each).  The block store's sample capture, pentago/end/block_store.cc:
+ * 190-194, with the index map of pentago/end/check.cc:110-121.  Synchronous. */
+pe_status pe_end_slab_samples(const pe_slab* slab, int n, const uint64_t* boards, void* h_nodes);
+
+/* All standardized sections with `slice` stones that `root` depends on, sorted by sig():
+ * slice `slice` of descendent_sections (pentago/end/sections.cc:18-49).  Writes at most cap
+ * sections and the full count to *count (call with cap 0 to size). */
+pe_status pe_descendent_sections(const pe_section* root, int slice, int cap, pe_section* out, int* count);
+
+/* A slice store: whole sections of one slice resident on the current device, one slab per
+ * section covering all its blocks -- the one-GPU form of the block store the reference's
+ * slice loop fills (pentago/mpi/toplevel.cc:263-421, pentago/end/block_store.h:113-160). */
+typedef struct pe_slice pe_slice;
+/* Allocate device slabs for `n` sections with the same stone count (<= 35). */
+pe_status pe_slice_create(int n, const pe_section* sections, pe_slice** out);
+/* Device bytes the slabs of these sections take (no allocation). */
+int64_t pe_slice_bytes(int n, const pe_section* sections);
+/* Compute every section of `out` from `children` (the store of the next slice holding every
+ * child section they read; NULL when `out` is slice 35): one fused plan over all sections
+ * (line passes at slice 35), run on `stream` and waited for. */
+pe_status pe_slice_compute(pe_slice* out, const pe_slice* children, void* stream);
+/* Section k's slab (k < sections), valid until pe_slice_destroy. */
+pe_status pe_slice_slab(const pe_slice* s, int k, pe_slab* slab);
+int pe_slice_sections(const pe_slice* s);
+/* pe_end_count_slabs over every section of the store. */
+pe_status pe_slice_counts(const pe_slice* s, uint64_t (*counts)[4]);
+pe_status pe_slice_destroy(pe_slice* s);
+
 /* ---------------- midsolve ---------------- */
 
 /* high_board_s: pentago/high/board_c.h:9-12 */

@@ -92,6 +92,8 @@ def lib():
             "po_compute_lines_threaded": (C.c_int, [C.c_int, P(Line), P(C.c_void_p), P(C.c_void_p), C.c_int]),
             "po_eval_reset": (None, []),
             "po_eval_node": (C.c_int, [Section, P(C.c_int), P(Super)]),
+            "po_popcounts_over_stabilizers": (C.c_int, [C.c_uint64, P(Super), P(C.c_uint16)]),
+            "po_section_counts": (C.c_int, [Section, P(C.c_uint64)]),
             "po_brute_value": (C.c_int, [C.c_uint64, C.c_uint64]),
             "po_brute_super": (None, [C.c_uint64, C.c_uint64, P(Super)]),
             "po_half_rmax": (Half, [Half]),
@@ -291,6 +293,23 @@ def eval_node(counts, index):
     idx = (C.c_int * 4)(*index)
     assert lib().po_eval_node(section(counts), idx, out) == 0
     return super_to_int(out[0]), super_to_int(out[1])
+
+
+def popcounts_over_stabilizers(board, wins, notlose):
+    """(wins, wins-or-ties, total) over distinct rotated positions; None where the reference
+    asserts (counts not multiples of the stabilizer size)."""
+    v = (Super * 2)(super_from_int(wins), super_from_int(notlose))
+    out = (C.c_uint16 * 3)()
+    r = lib().po_popcounts_over_stabilizers(board, v, out)
+    return None if r else tuple(out)
+
+
+def count_section(counts):
+    """(wins, wins-or-ties, total) summed over every node of a standard section."""
+    out = (C.c_uint64 * 3)()
+    r = lib().po_section_counts(section(counts), out)
+    assert r == 0, r
+    return tuple(out)
 
 
 def brute_super(me, them):

@@ -109,6 +109,15 @@ int po_compute_microlines(const po_line* line, const po_super* input, po_super* 
 void po_eval_reset(void);
 int po_eval_node(po_section standard, const int index[4], po_super out[2]);
 
+/* ---- counts (pentago/base/count.cc, pentago/end/block_store.cc) ---- */
+/* (wins, wins-or-ties, total) of one node over its distinct rotated positions:
+ * popcounts_over_stabilizers (pentago/base/count.cc:235-262).  Returns -1 where the reference
+ * asserts (popcounts not multiples of the stabilizer size). */
+int po_popcounts_over_stabilizers(uint64_t board, const po_super wins[2], uint16_t out[3]);
+/* Sum over every node of a standard section, node values from po_eval_node: count_block_wins
+ * (pentago/end/block_store.cc:330-349) summed over the section's blocks. */
+int po_section_counts(po_section s, uint64_t out[3]);
+
 /* ---- brute force (tests only) ---- */
 /* value in {-1,0,1} of side-to-move `me` vs `them`, next action a stone placement. */
 int po_brute_value(uint64_t me, uint64_t them);

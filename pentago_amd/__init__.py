@@ -151,6 +151,17 @@ def lib():
             "pe_end_plan_nodes": (C.c_int64, [V]),
             "pe_end_plan_destroy": (C.c_int, [V]),
             "pe_fill_seeded": (C.c_int, [V, C.c_int64, C.c_uint64, C.c_int64, V]),
+            "pe_end_count_blocks": (C.c_int, [P(Slab), V, V]),
+            "pe_end_count_slabs": (C.c_int, [C.c_int, P(Slab), V]),
+            "pe_end_slab_samples": (C.c_int, [P(Slab), C.c_int, P(C.c_uint64), V]),
+            "pe_descendent_sections": (C.c_int, [P(Section), C.c_int, C.c_int, P(Section), P(C.c_int)]),
+            "pe_slice_create": (C.c_int, [C.c_int, P(Section), P(V)]),
+            "pe_slice_bytes": (C.c_int64, [C.c_int, P(Section)]),
+            "pe_slice_compute": (C.c_int, [V, V, V]),
+            "pe_slice_slab": (C.c_int, [V, C.c_int, P(Slab)]),
+            "pe_slice_sections": (C.c_int, [V]),
+            "pe_slice_counts": (C.c_int, [V, V]),
+            "pe_slice_destroy": (C.c_int, [V]),
             "pe_midsolve_workspace_size": (C.c_int64, [C.c_int]),
             "pe_midsolve_internal": (C.c_int, [P(HighBoard), P(HalfSupers)]),
             "pe_midsolve": (C.c_int, [P(HighBoard), P(MidValue), P(C.c_int)]),
@@ -315,6 +326,103 @@ class Plan:
 
 def fill_seeded(d_nodes, nodes, seed, first_node=0, stream=None):
     _check(lib().pe_fill_seeded(d_nodes, nodes, seed, first_node, stream), "pe_fill_seeded")
+
+
+def count_slabs(slabs):
+    """count_block_wins summed per slab: (n, 4) uint64 array of (wins, wins-or-ties, total,
+    nodes not invariant under their stabilizer) for the player to move."""
+    slabs = list(slabs)
+    out = np.zeros((len(slabs), 4), dtype=np.uint64)
+    arr = (Slab * max(len(slabs), 1))(*slabs)
+    _check(lib().pe_end_count_slabs(len(slabs), arr, out.ctypes.data), "pe_end_count_slabs")
+    return out
+
+
+def count_blocks(slab, d_counts, stream=None):
+    """Per-block counts of one slab into device memory (4 uint64 per block), asynchronous."""
+    _check(lib().pe_end_count_blocks(C.byref(slab), d_counts, stream), "pe_end_count_blocks")
+
+
+def slab_samples(slab, boards):
+    """Nodes of rotation-standardized boards inside a slab: (n, 8) uint64 (win words, then
+    win-or-tie words, for the player to move)."""
+    b = np.ascontiguousarray(boards, dtype=np.uint64)
+    out = np.zeros((len(b), 8), dtype=np.uint64)
+    _check(lib().pe_end_slab_samples(C.byref(slab), len(b), b.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                      out.ctypes.data), "pe_end_slab_samples")
+    return out
+
+
+def descendent_sections(root, slice_):
+    """Standardized sections of a slice that root depends on, sorted by sig
+    (pentago/end/sections.cc:18-49)."""
+    r = Section.of(root)
+    n = C.c_int()
+    _check(lib().pe_descendent_sections(C.byref(r), slice_, 0, None, C.byref(n)), "pe_descendent_sections")
+    arr = (Section * max(n.value, 1))()
+    _check(lib().pe_descendent_sections(C.byref(r), slice_, n.value, arr, C.byref(n)), "pe_descendent_sections")
+    return [arr[k].counts for k in range(n.value)]
+
+
+def section_sig(counts):
+    """section_t::sig() (pentago/base/section.h:40-48)."""
+    return sum(int(counts[q][0]) << (16 * q) | int(counts[q][1]) << (16 * q + 8) for q in range(4))
+
+
+def counts_rows(sections, counts, slice_):
+    """Rows (sig, black wins, white wins, total) as the reference writes counts-<slice>.npy
+    (pentago/mpi/io.cc:573-581): wins are the mover's, losses = total - wins-or-ties, and
+    odd slices swap them so wins are black's."""
+    rows = np.zeros((len(sections), 4), dtype=np.uint64)
+    for k, sec in enumerate(sections):
+        wins, losses = int(counts[k][0]), int(counts[k][2]) - int(counts[k][1])
+        if slice_ & 1:
+            wins, losses = losses, wins
+        rows[k] = (section_sig(sec), wins, losses, int(counts[k][2]))
+    return rows
+
+
+def slice_bytes(sections):
+    arr = (Section * max(len(sections), 1))(*[Section.of(c) for c in sections])
+    return lib().pe_slice_bytes(len(sections), arr)
+
+
+class SliceStore:
+    """Whole sections of one slice resident on the device (pe_slice): compute() fills them
+    from the next slice's store, counts() gives count_block_wins per section."""
+
+    def __init__(self, sections):
+        self.sections = [tuple(tuple(c) for c in s) for s in sections]
+        arr = (Section * len(self.sections))(*[Section.of(c) for c in self.sections])
+        h = C.c_void_p()
+        _check(lib().pe_slice_create(len(self.sections), arr, C.byref(h)), "pe_slice_create")
+        self.handle = h
+        self.slice = sum(sum(c) for c in self.sections[0])
+        self.index = {s: k for k, s in enumerate(self.sections)}
+
+    def compute(self, children=None, stream=None):
+        _check(lib().pe_slice_compute(self.handle, children.handle if children else None, stream), "pe_slice_compute")
+
+    def slab(self, k):
+        out = Slab()
+        _check(lib().pe_slice_slab(self.handle, k, C.byref(out)), "pe_slice_slab")
+        return out
+
+    def counts(self):
+        out = np.zeros((len(self.sections), 4), dtype=np.uint64)
+        _check(lib().pe_slice_counts(self.handle, out.ctypes.data), "pe_slice_counts")
+        return out
+
+    def close(self):
+        if self.handle:
+            lib().pe_slice_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------- midsolve ----------------

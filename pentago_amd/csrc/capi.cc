@@ -59,15 +59,25 @@ pe_status device_tables(DevTables* out, std::string* err) {
   std::vector<int32_t> choose(32 * 32);
   for (int n = 0; n < 32; n++)
     for (int k = 0; k < 32; k++) choose[n * 32 + k] = choose_host(n, k);
+  // log2 of the number of local rotations fixing a quadrant: 2 if a quarter turn fixes both
+  // sides, 1 if only a half turn does, else 0 (pentago/base/count.cc:235-245)
+  std::vector<uint8_t> stab(T->rmin_flat_size);
+  for (int i = 0; i < T->rmin_flat_size; i++) {
+    const int s0 = T->unpack[T->rmin_flat[i]][0], s1 = T->unpack[T->rmin_flat[i]][1];
+    const int r0 = T->rot[s0][0], r1 = T->rot[s1][0];
+    stab[i] = (r0 == s0 && r1 == s1) ? 2 : (T->rot[r0][0] == s0 && T->rot[r1][0] == s1) ? 1 : 0;
+  }
   const size_t sizes[] = {sizeof T->superwin, sizeof(uint16_t) * size_t(T->rmin_flat_size),
                           sizeof(uint32_t) * sides.size(), sizeof T->rmin_inverse, sizeof T->pack,
-                          sizeof T->commute, sizeof(int32_t) * choose.size()};
-  const void* srcs[] = {T->superwin, T->rmin_flat, sides.data(), T->rmin_inverse, T->pack, T->commute, choose.data()};
-  size_t total = 0, offs[7];
-  for (int i = 0; i < 7; i++) { offs[i] = total; total += (sizes[i] + 255) & ~size_t(255); }
+                          sizeof T->commute, sizeof(int32_t) * choose.size(), stab.size()};
+  const void* srcs[] = {T->superwin, T->rmin_flat, sides.data(), T->rmin_inverse, T->pack, T->commute, choose.data(),
+                        stab.data()};
+  constexpr int kParts = 8;
+  size_t total = 0, offs[kParts];
+  for (int i = 0; i < kParts; i++) { offs[i] = total; total += (sizes[i] + 255) & ~size_t(255); }
   void* d = nullptr;
   if (hipMalloc(&d, total) != hipSuccess) { *err = "hipMalloc(tables) failed"; return PE_ERR_DEVICE; }
-  for (int i = 0; i < 7; i++)
+  for (int i = 0; i < kParts; i++)
     if (hipMemcpy(static_cast<uint8_t*>(d) + offs[i], srcs[i], sizes[i], hipMemcpyHostToDevice) != hipSuccess) {
       *err = "hipMemcpy(tables) failed";
       return PE_ERR_DEVICE;
@@ -81,6 +91,7 @@ pe_status device_tables(DevTables* out, std::string* err) {
   D.pack = reinterpret_cast<const uint16_t*>(b + offs[4]);
   D.commute = reinterpret_cast<const uint8_t*>(b + offs[5]);
   D.choose = reinterpret_cast<const int32_t*>(b + offs[6]);
+  D.rmin_stab = b + offs[7];
   g_dev[dev] = D;
   g_dev_ready[dev] = true;
   *out = D;
@@ -184,9 +195,8 @@ pe_status pe_end_compute_lines(int n, const pe_line* lines, const void* const* d
 }
 
 struct pe_end_plan {
-  std::vector<EndPlan> passes;   // line plans run in order
-  FusedPlan fused;               // or one fused block plan
-  bool is_fused = false;
+  FusedPlan fused;               // fused block plan (run first when present)
+  std::vector<EndPlan> passes;   // then line plans, in order
   int64_t nodes = 0;
 };
 
@@ -239,20 +249,26 @@ pe_status pe_end_slabs_plan_create(int noutputs, const pe_slab* outputs, int nch
   auto* plan = new pe_end_plan();
   std::string err;
   pe_status s = PE_OK;
-  if (mode == 0) {
-    plan->is_fused = true;
-    s = fused_plan_create(noutputs, outputs, nchildren, children, &plan->fused, &err);
+  // mode 0: sections with at most 34 stones go to the fused kernel; 35-stone sections (one
+  // open dimension, no child data) always run as line passes
+  std::vector<pe_slab> fused, lined;
+  for (int k = 0; k < noutputs; k++) {
+    int sum = 0;
+    for (int q = 0; q < 4; q++) sum += outputs[k].section.counts[q][0] + outputs[k].section.counts[q][1];
+    (mode == 0 && sum <= 34 ? fused : lined).push_back(outputs[k]);
+  }
+  if (!fused.empty()) {
+    s = fused_plan_create(int(fused.size()), fused.data(), nchildren, children, &plan->fused, &err);
     plan->nodes = plan->fused.output_nodes;
-  } else {
-    plan->nodes = 0;
-    for (int k = 0; k < noutputs && s == PE_OK; k++) {
-      int64_t nodes = 0;
-      s = slab_line_passes(outputs[k], nchildren, children, &plan->passes, &nodes, &err);
-      plan->nodes += nodes;
-    }
+  }
+  for (size_t k = 0; k < lined.size() && s == PE_OK; k++) {
+    int64_t nodes = 0;
+    s = slab_line_passes(lined[k], nchildren, children, &plan->passes, &nodes, &err);
+    plan->nodes += nodes;
   }
   if (s != PE_OK) {
     for (auto& p : plan->passes) plan_destroy(&p);
+    fused_plan_destroy(&plan->fused);
     delete plan;
     return fail(s, err.empty() ? pe_last_error() : err);
   }
@@ -263,7 +279,10 @@ pe_status pe_end_slabs_plan_create(int noutputs, const pe_slab* outputs, int nch
 pe_status pe_end_plan_run(const pe_end_plan* plan, void* stream) {
   if (!plan) return fail(PE_ERR_INVALID, "null plan");
   const hipStream_t st = static_cast<hipStream_t>(stream);
-  if (plan->is_fused) return fused_plan_run(plan->fused, st);
+  if (plan->fused.nblocks) {
+    const pe_status s = fused_plan_run(plan->fused, st);
+    if (s != PE_OK) return s;
+  }
   for (const auto& p : plan->passes) {
     const pe_status s = plan_run(p, st);
     if (s != PE_OK) return s;

@@ -252,7 +252,7 @@ static pe_status slab_lines(const pe_slab& out, int nch, const pe_slab* ch, std:
   std::memcpy(s.c, out.section.counts, 8);
   int shape[4];
   if (!s.valid_counts() || !s.shape(shape)) { *err = "invalid output section"; return PE_ERR_INVALID; }
-  if (s.sum() > 34) { *err = "slab plans need sections with at most 34 stones"; return PE_ERR_INVALID; }
+  if (s.sum() > 35) { *err = "slab plans need sections with at most 35 stones"; return PE_ERR_INVALID; }
   for (int a = 0; a < 4; a++)
     if (out.lo[a] < 0 || out.hi[a] <= out.lo[a] || 8 * (out.hi[a] - 1) >= shape[a]) { *err = "output box outside the section"; return PE_ERR_INVALID; }
   for (int d = 0; d < 4; d++) {
@@ -270,6 +270,11 @@ static pe_status slab_lines(const pe_slab& out, int nch, const pe_slab* ch, std:
           LineHost h;
           const pe_status st = make_line(L, &h, err);
           if (st != PE_OK) return st;
+          if (h.dev.slice35) {   // the game ends after the move: no child data
+            (*index)[std::make_tuple(d, x, y, z)] = int(lines->size());
+            lines->push_back(std::move(h));
+            continue;
+          }
           const int cd = h.info.child_dimension;
           const pe_slab* found = nullptr;
           for (int c = 0; c < nch && !found; c++) {
@@ -312,6 +317,7 @@ pe_status fused_plan_create(int nout, const pe_slab* outs, int nch, const pe_sla
     if (st != PE_OK) return st;
     Section s;
     std::memcpy(s.c, out.section.counts, 8);
+    if (s.sum() > 34) { *err = "the fused kernel needs sections with at most 34 stones"; return PE_ERR_INVALID; }
     SecDev sec;
     std::memset(&sec, 0, sizeof sec);
     sec.turn = s.sum() & 1;

@@ -1,0 +1,242 @@
+// Slice stores, section enumeration, block counts and sparse samples: the one-GPU form of
+// the reference's slice loop (pentago/mpi/toplevel.cc:263-421) and block store
+// (pentago/end/block_store.h:113-160), without MPI, compression or files.
+//
+// A store holds whole sections of one slice on the device, one slab per section covering
+// every block (a 4096-node slot per block).  Computing a slice is one slab plan over all of
+// its sections (the fused kernel; line passes at slice 35) reading the next slice's store.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "device.h"
+#include "end.h"
+#include "tables.h"
+
+namespace pe {
+pe_status count_slab_blocks(const pe_slab& slab, uint64_t* d_counts, hipStream_t stream, std::string* err);
+int64_t slab_blocks(const pe_slab& slab);
+
+namespace {
+
+pe_status failed(pe_status s, const std::string& e) {
+  set_error(e);
+  return s;
+}
+
+bool section_of(const pe_section& p, Section* s, int shape[4]) {
+  std::memcpy(s->c, p.counts, 8);
+  return s->valid_counts() && s->shape(shape);
+}
+
+pe_slab whole_slab(const pe_section& sec, const int shape[4], void* data) {
+  pe_slab slab;
+  slab.section = sec;
+  for (int a = 0; a < 4; a++) {
+    slab.lo[a] = 0;
+    slab.hi[a] = (shape[a] + 7) / 8;
+  }
+  slab.data = data;
+  return slab;
+}
+
+// Counts of n slabs summed per slab, through one device buffer and one copy back.
+pe_status count_slabs(int n, const pe_slab* slabs, uint64_t (*counts)[4]) {
+  std::vector<int64_t> first(n + 1, 0);
+  for (int k = 0; k < n; k++) first[k + 1] = first[k] + slab_blocks(slabs[k]);
+  std::fill(&counts[0][0], &counts[0][0] + 4 * size_t(n), uint64_t(0));
+  if (!first[n]) return PE_OK;
+  void* d = nullptr;
+  if (hipMalloc(&d, sizeof(uint64_t) * 4 * first[n]) != hipSuccess) return failed(PE_ERR_DEVICE, "hipMalloc(counts) failed");
+  std::string err;
+  pe_status s = PE_OK;
+  for (int k = 0; k < n && s == PE_OK; k++)
+    s = count_slab_blocks(slabs[k], static_cast<uint64_t*>(d) + 4 * first[k], nullptr, &err);
+  std::vector<uint64_t> h(4 * first[n]);
+  if (s == PE_OK && hipMemcpy(h.data(), d, sizeof(uint64_t) * h.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+    s = PE_ERR_DEVICE;
+    err = "counting blocks failed";
+  }
+  (void)hipFree(d);
+  if (s != PE_OK) return failed(s, err);
+  for (int k = 0; k < n; k++)
+    for (int64_t b = first[k]; b < first[k + 1]; b++)
+      for (int j = 0; j < 4; j++) counts[k][j] += h[4 * b + j];
+  return PE_OK;
+}
+
+}  // namespace
+}  // namespace pe
+
+using namespace pe;
+
+struct pe_slice {
+  std::vector<pe_slab> slabs;
+  std::vector<void*> data;
+  int slice = 0;
+  int device = 0;
+};
+
+extern "C" {
+
+pe_status pe_end_count_blocks(const pe_slab* slab, uint64_t* d_counts, void* stream) {
+  if (!slab) return failed(PE_ERR_INVALID, "null slab");
+  std::string err;
+  const pe_status s = count_slab_blocks(*slab, d_counts, static_cast<hipStream_t>(stream), &err);
+  return s == PE_OK ? s : failed(s, err);
+}
+
+pe_status pe_end_count_slabs(int n, const pe_slab* slabs, uint64_t (*counts)[4]) {
+  if (n < 0 || (n && (!slabs || !counts))) return failed(PE_ERR_INVALID, "bad count arguments");
+  return count_slabs(n, slabs, counts);
+}
+
+pe_status pe_end_slab_samples(const pe_slab* slab, int n, const uint64_t* boards, void* h_nodes) {
+  if (!slab || n < 0 || (n && (!boards || !h_nodes))) return failed(PE_ERR_INVALID, "bad sample arguments");
+  Section s;
+  int shape[4];
+  if (!section_of(slab->section, &s, shape)) return failed(PE_ERR_INVALID, "invalid slab section");
+  const Tables& T = tables();
+  for (int i = 0; i < n; i++) {
+    int idx[4];
+    for (int q = 0; q < 4; q++) {
+      const int quad = int(boards[i] >> (16 * q) & 0xffff);
+      if (quad >= 19683) return failed(PE_ERR_INVALID, "sample " + std::to_string(i) + ": invalid quadrant");
+      const int black = __builtin_popcount(T.unpack[quad][0]), white = __builtin_popcount(T.unpack[quad][1]);
+      if (black != s.c[q][0] || white != s.c[q][1])
+        return failed(PE_ERR_INVALID, "sample " + std::to_string(i) + ": board is not in the slab's section");
+      const int ir = T.rmin_inverse[quad];
+      if (ir & 3) return failed(PE_ERR_INVALID, "sample " + std::to_string(i) + ": board is not rotation standardized");
+      idx[q] = ir >> 2;
+    }
+    int b[4], n_[4];
+    int64_t slot = 0, within = 0;
+    for (int a = 0; a < 4; a++) {
+      b[a] = idx[a] / 8;
+      if (b[a] < slab->lo[a] || b[a] >= slab->hi[a])
+        return failed(PE_ERR_INVALID, "sample " + std::to_string(i) + ": block outside the slab's box");
+      n_[a] = std::min(8, shape[a] - 8 * b[a]);
+      slot = slot * (slab->hi[a] - slab->lo[a]) + (b[a] - slab->lo[a]);
+      within = within * n_[a] + idx[a] % 8;
+    }
+    const auto* src = static_cast<const uint8_t*>(slab->data) + 64 * (4096 * slot + within);
+    if (hipMemcpy(static_cast<uint8_t*>(h_nodes) + 64 * size_t(i), src, 64, hipMemcpyDeviceToHost) != hipSuccess)
+      return failed(PE_ERR_DEVICE, "sample copy failed");
+  }
+  return PE_OK;
+}
+
+pe_status pe_descendent_sections(const pe_section* root, int slice, int cap, pe_section* out, int* count) {
+  if (!root || !count || cap < 0 || (cap && !out)) return failed(PE_ERR_INVALID, "bad section arguments");
+  Section r;
+  int shape[4];
+  if (!section_of(*root, &r, shape)) return failed(PE_ERR_INVALID, "invalid root section");
+  if (slice < r.sum() || slice > 35) return failed(PE_ERR_INVALID, "slice outside [root stones, 35]");
+  // depth-first over standardized sections (every child adds one stone of the player to move)
+  std::set<uint64_t> seen;
+  std::vector<Section> stack{r};
+  std::vector<Section> found;
+  while (!stack.empty()) {
+    Section s;
+    stack.back().standardize(&s);
+    stack.pop_back();
+    if (!seen.insert(s.sig()).second) continue;
+    if (s.sum() == slice) {
+      found.push_back(s);
+      continue;
+    }
+    for (int q = 0; q < 4; q++)
+      if (s.c[q][0] + s.c[q][1] < 9) stack.push_back(s.child(q));
+  }
+  std::sort(found.begin(), found.end(), [](const Section& a, const Section& b) { return a.sig() < b.sig(); });
+  *count = int(found.size());
+  for (int k = 0; k < std::min(cap, int(found.size())); k++) std::memcpy(out[k].counts, found[k].c, 8);
+  return PE_OK;
+}
+
+int64_t pe_slice_bytes(int n, const pe_section* sections) {
+  if (n < 0 || (n && !sections)) return -1;
+  int64_t bytes = 0;
+  for (int k = 0; k < n; k++) {
+    Section s;
+    int shape[4];
+    if (!section_of(sections[k], &s, shape)) return -1;
+    int64_t blocks = 1;
+    for (int a = 0; a < 4; a++) blocks *= (shape[a] + 7) / 8;
+    bytes += blocks * 4096 * 64;
+  }
+  return bytes;
+}
+
+pe_status pe_slice_create(int n, const pe_section* sections, pe_slice** out) {
+  if (n <= 0 || !sections || !out) return failed(PE_ERR_INVALID, "bad slice arguments");
+  auto* st = new pe_slice();
+  (void)hipGetDevice(&st->device);
+  for (int k = 0; k < n; k++) {
+    Section s, t;
+    int shape[4];
+    if (!section_of(sections[k], &s, shape) || s.sum() > 35 || (k && s.sum() != st->slice)) {
+      pe_slice_destroy(st);
+      return failed(PE_ERR_INVALID, "section " + std::to_string(k) + " is invalid or of another slice");
+    }
+    s.standardize(&t);
+    if (t.sig() != s.sig()) {
+      pe_slice_destroy(st);
+      return failed(PE_ERR_INVALID, "section " + std::to_string(k) + " is not standardized");
+    }
+    st->slice = s.sum();
+    int64_t blocks = 1;
+    for (int a = 0; a < 4; a++) blocks *= (shape[a] + 7) / 8;
+    void* d = nullptr;
+    if (hipMalloc(&d, size_t(blocks) * 4096 * 64) != hipSuccess) {
+      pe_slice_destroy(st);
+      return failed(PE_ERR_CAPACITY, "hipMalloc of section " + std::to_string(k) + " failed");
+    }
+    st->data.push_back(d);
+    st->slabs.push_back(whole_slab(sections[k], shape, d));
+  }
+  *out = st;
+  return PE_OK;
+}
+
+pe_status pe_slice_compute(pe_slice* out, const pe_slice* children, void* stream) {
+  if (!out) return failed(PE_ERR_INVALID, "null slice");
+  if (out->slice < 35 && !children) return failed(PE_ERR_INVALID, "slices below 35 need the next slice's store");
+  if (children && children->slice != out->slice + 1) return failed(PE_ERR_INVALID, "children are not the next slice");
+  pe_end_plan* plan = nullptr;
+  const int nch = out->slice < 35 ? int(children->slabs.size()) : 0;
+  pe_status s = pe_end_slabs_plan_create(int(out->slabs.size()), out->slabs.data(), nch,
+                                         nch ? children->slabs.data() : nullptr, 0, &plan);
+  if (s != PE_OK) return s;
+  s = pe_end_plan_run(plan, stream);
+  if (s == PE_OK && hipStreamSynchronize(static_cast<hipStream_t>(stream)) != hipSuccess)
+    s = failed(PE_ERR_DEVICE, "slice compute failed");
+  pe_end_plan_destroy(plan);
+  return s;
+}
+
+pe_status pe_slice_slab(const pe_slice* s, int k, pe_slab* slab) {
+  if (!s || !slab || k < 0 || k >= int(s->slabs.size())) return failed(PE_ERR_INVALID, "bad slab index");
+  *slab = s->slabs[k];
+  return PE_OK;
+}
+
+int pe_slice_sections(const pe_slice* s) { return s ? int(s->slabs.size()) : -1; }
+
+pe_status pe_slice_counts(const pe_slice* s, uint64_t (*counts)[4]) {
+  if (!s || !counts) return failed(PE_ERR_INVALID, "null argument");
+  return count_slabs(int(s->slabs.size()), s->slabs.data(), counts);
+}
+
+pe_status pe_slice_destroy(pe_slice* s) {
+  if (!s) return PE_OK;
+  for (void* d : s->data) (void)hipFree(d);
+  delete s;
+  return PE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,175 @@
+"""Whole slices on the GPU against the production run.
+
+Slices 35, 34 and 33 (every section the empty board reaches, pentago/end/sections.cc:18-49)
+are computed on the device in slice stores -- slice 35 from nothing, 34 from 35, 33 from 34
+in groups that fit next to slice 34 -- and each section's count_block_wins total
+(pentago/end/block_store.cc:330-349) is compared, converted as pentago/mpi/io.cc:573-581,
+with the reference's data/counts-<slice>.npy rows (the production run's counts, committed
+as fixtures).  The production sparse samples of each slice (pentago/high/high_test.cc:41-58)
+are read out of the stores (pentago/end/check.cc:149-158 format).  The count kernel itself
+is checked on seeded data, stabilized boards included, against the oracle's
+popcounts_over_stabilizers.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import pentago_amd as P
+from pentago_amd import hip
+from oracle import oracle as O
+from tests.common import golden
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+EMPTY = ((0, 0),) * 4
+_stores = {}
+
+
+def golden_rows(slice_):
+    rows = np.load(os.path.join(GOLDEN, "counts-%d.npy" % slice_), allow_pickle=False)
+    return rows[np.argsort(rows[:, 0], kind="stable")]
+
+
+def board_section(board):
+    counts = []
+    for q in range(4):
+        quad = (board >> (16 * q)) & 0xffff
+        b = w = 0
+        for _ in range(9):
+            d = quad % 3
+            quad //= 3
+            b += d == 1
+            w += d == 2
+        counts.append((b, w))
+    return tuple(counts)
+
+
+def check_samples(slice_, slab_of):
+    """Production samples whose section slab_of(section) returns (None: not resident)."""
+    data = golden("samples.json")[str(slice_)]
+    checked = 0
+    full = (1 << 256) - 1
+    for board, wins in zip(data["boards"], data["wins"]):
+        slab = slab_of(board_section(board))
+        if slab is None:
+            continue
+        node = P.slab_samples(slab, [board])[0]
+        win = sum(int(node[i]) << (64 * i) for i in range(4))
+        wot = sum(int(node[4 + i]) << (64 * i) for i in range(4))
+        black, white = win, full & ~wot
+        if slice_ & 1:
+            black, white = white, black
+        assert black == sum(wins[i] << (64 * i) for i in range(4)), board
+        assert white == sum(wins[4 + i] << (64 * i) for i in range(4)), board
+        checked += 1
+    return checked
+
+
+def store(slice_):
+    if slice_ not in _stores:
+        secs = P.descendent_sections(EMPTY, slice_)
+        s = P.SliceStore(secs)
+        s.compute(store(slice_ + 1) if slice_ < 35 else None)
+        _stores[slice_] = s
+    return _stores[slice_]
+
+
+def release(slice_):
+    s = _stores.pop(slice_, None)
+    if s is not None:
+        s.close()
+
+
+def test_count_blocks_seeded_against_oracle():
+    # (2,0) has opposite pairs (fixed by a half turn) and (0,1) the centre stone (fixed by a
+    # quarter turn), so node stabilizers have 1, 2, 4, 8, 16 and 32 elements, and seeded data
+    # is mostly not invariant under them (counted in column 3)
+    counts = ((2, 0), (1, 1), (0, 1), (0, 1))
+    shape = P.section_shape(counts)
+    nb = [(s + 7) // 8 for s in shape]
+    blocks = int(np.prod(nb))
+    buf = hip.Buffer(64 * 4096 * blocks)
+    P.fill_seeded(buf.ptr, 4096 * blocks, 77, 0)
+    slab = P.Slab.of(counts, [0] * 4, nb, buf.ptr)
+    out = hip.Buffer(32 * blocks)
+    P.count_blocks(slab, out.ptr)
+    hip.synchronize()
+    got = out.download(np.zeros(4 * blocks, dtype=np.uint64)).reshape(blocks, 4)
+    data = buf.download(np.zeros(8 * 4096 * blocks, dtype=np.uint64)).reshape(blocks, 4096, 8)
+    bits = np.unpackbits(data.view(np.uint8), axis=2).reshape(blocks, 4096, 2, 256).sum(axis=3)
+    rm = [O.rmin(*counts[q])[0] for q in range(4)]
+    full = (1 << 256) - 1
+    want = np.zeros((blocks, 4), dtype=np.uint64)
+    shifts = set()
+    for k in range(blocks):
+        b = np.unravel_index(k, nb)
+        bs = [min(8, shape[a] - 8 * b[a]) for a in range(4)]
+        for p in range(int(np.prod(bs))):
+            i = np.unravel_index(p, bs)
+            board = sum(int(rm[q][8 * b[q] + i[q]]) << (16 * q) for q in range(4))
+            total = O.popcounts_over_stabilizers(board, full, full)[2]
+            shift = (256 // total).bit_length() - 1
+            shifts.add(shift)
+            w0, w1 = int(bits[k, p, 0]), int(bits[k, p, 1])
+            res = O.popcounts_over_stabilizers(
+                board, sum(int(data[k, p, j]) << (64 * j) for j in range(4)),
+                sum(int(data[k, p, 4 + j]) << (64 * j) for j in range(4)))
+            bad = ((w0 | w1) & ((1 << shift) - 1)) != 0
+            assert (res is None) == bad
+            want[k] += np.array([w0 >> shift, w1 >> shift, total, int(bad)], dtype=np.uint64)
+    assert np.array_equal(got, want)
+    assert shifts == {0, 1, 2, 3, 4, 5}
+    assert 0 < int(want[:, 3].sum()) < int(np.prod(shape))
+    # the synchronous per-slab sum agrees
+    assert np.array_equal(P.count_slabs([slab])[0], want.sum(axis=0))
+
+
+def test_slice_35_matches_production_counts():
+    s = store(35)
+    counts = s.counts()
+    assert int(counts[:, 3].sum()) == 0
+    assert np.array_equal(P.counts_rows(s.sections, counts, 35), golden_rows(35))
+    assert check_samples(35, lambda sec: s.slab(s.index[sec]) if sec in s.index else None) == 32
+
+
+def test_slice_34_matches_production_counts():
+    s = store(34)
+    release(35)
+    counts = s.counts()
+    assert int(counts[:, 3].sum()) == 0
+    assert np.array_equal(P.counts_rows(s.sections, counts, 34), golden_rows(34))
+    assert check_samples(34, lambda sec: s.slab(s.index[sec]) if sec in s.index else None) == 32
+
+
+def test_slice_33_matches_production_counts():
+    """Slice 33 (518 GB as slabs) in groups of sections that fit beside slice 34."""
+    children = store(34)
+    secs = P.descendent_sections(EMPTY, 33)
+    free, _ = hip.mem_info()
+    budget = free - 16e9
+    assert budget > 20e9, "not enough device memory beside slice 34"
+    groups, cur, used = [], [], 0
+    for sec in secs:
+        b = P.slice_bytes([sec])
+        if cur and used + b > budget:
+            groups.append(cur)
+            cur, used = [], 0
+        cur.append(sec)
+        used += b
+    groups.append(cur)
+    rows, samples = [], 0
+    for g in groups:
+        st = P.SliceStore(g)
+        try:
+            st.compute(children)
+            counts = st.counts()
+            assert int(counts[:, 3].sum()) == 0
+            rows.append(P.counts_rows(st.sections, counts, 33))
+            samples += check_samples(33, lambda sec: st.slab(st.index[sec]) if sec in st.index else None)
+        finally:
+            st.close()
+    release(34)
+    assert np.array_equal(np.concatenate(rows), golden_rows(33))
+    assert samples == 32
