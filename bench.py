@@ -12,9 +12,11 @@ once (every node gets all its dimensions and is written once); mode 1 runs four 
 passes per box, OR-merging into the output.  Inputs are on the device before timing starts.
 
 Prints one JSON line.  Metric: superpositions/s = output nodes per second (the reference's
-output nodes/second, pentago/mpi/toplevel.cc:83-94).  The same line carries the batched
-midsolve (boards/s against plain per-board launches) and serving (p50/p99 latency at
-Poisson loads) measurements unless --no-mid / --no-serve.
+output nodes/second, pentago/mpi/toplevel.cc:83-94).  The same line carries a box of the
+slice-30 section, whole slices 35, 34 and 33 on real data (computed from nothing, counts
+checked against the production run's), the batched midsolve (boards/s against plain
+per-board launches) and serving (p50/p99 latency at Poisson loads) measurements unless
+--no-section30 / --no-slices / --no-mid / --no-serve.
 
 --copy-probe BYTES runs one plain device copy of BYTES bytes instead (a kernel of known
 traffic, used to scale the FETCH_SIZE / WRITE_SIZE counters).
@@ -212,6 +214,104 @@ def section30_bench(P, hip, steps):
     return out
 
 
+def whole_section_nodes(P, counts):
+    """(nodes of the section, nodes of the child sections its lines read, one per open dimension)."""
+    nodes = int(np.prod(P.section_shape(counts)))
+    turn = sum(map(sum, counts)) & 1
+    child = 0
+    for d in range(4):
+        if sum(counts[d]) < 9:
+            c = [list(x) for x in counts]
+            c[d][turn] += 1
+            child += int(np.prod(P.section_shape(tuple(map(tuple, c)))))
+    return nodes, child
+
+
+def slices_bench(P, hip, budget=None):
+    """Real data: slices 35, 34 and 33 (every section the empty board reaches) computed on
+    the device from nothing, slice 33 in groups of sections beside slice 34; kernel time
+    from HIP events on the launch stream, plan build time apart, the count kernel timed over
+    each slice, and every section's counts compared with the production run's."""
+    empty = ((0, 0),) * 4
+    golden = os.path.join(ROOT, "tests", "golden")
+    stream = hip.Stream()
+    ev0, ev1 = hip.Event(), hip.Event()
+    out, stores = {}, {}
+
+    def timed(fn):
+        ev0.record(stream)
+        fn()
+        ev1.record(stream)
+        ev1.synchronize()
+        return ev0.elapsed_ms(ev1) / 1e3
+
+    def run_group(secs, children):
+        st = P.SliceStore(secs)
+        slabs = [st.slab(k) for k in range(len(secs))]
+        t0 = time.time()
+        plan = P.Plan.slab(slabs, [children.slab(k) for k in range(len(children.sections))] if children else [], 0)
+        build = time.time() - t0
+        kernel = timed(lambda: plan.run(stream.ptr))
+        plan.close()
+        d_counts = hip.Buffer(32 * sum(s.blocks for s in slabs))
+        offs = np.cumsum([0] + [s.blocks for s in slabs])
+        count = timed(lambda: [P.count_blocks(s, d_counts.ptr + 32 * int(o), stream.ptr) for s, o in zip(slabs, offs)])
+        per_block = d_counts.download(np.zeros((int(offs[-1]), 4), dtype=np.uint64))
+        counts = np.array([per_block[offs[k]:offs[k + 1]].sum(axis=0) for k in range(len(slabs))], dtype=np.uint64)
+        return st, counts, build, kernel, count
+
+    for slice_ in (35, 34, 33):
+        secs = P.descendent_sections(empty, slice_)
+        want = np.load(os.path.join(golden, "counts-%d.npy" % slice_), allow_pickle=False)
+        want = want[np.argsort(want[:, 0], kind="stable")]
+        nodes = child = 0
+        for s in secs:
+            n, c = whole_section_nodes(P, s)
+            nodes += n
+            child += c if slice_ < 35 else 0
+        if slice_ > 33:
+            groups = [secs]
+        else:
+            free, _ = hip.mem_info()
+            cap = (budget or free - 16e9)
+            groups, cur, used = [], [], 0
+            for s in secs:
+                b = P.slice_bytes([s])
+                if cur and used + b > cap:
+                    groups.append(cur)
+                    cur, used = [], 0
+                cur.append(s)
+                used += b
+            groups.append(cur)
+        rows, build = [], 0.0
+        kernel = count = 0.0
+        asym = 0
+        for g in groups:
+            st, counts, b, k, c = run_group(g, stores.get(slice_ + 1))
+            build, kernel, count = build + b, kernel + k, count + c
+            asym += int(counts[:, 3].sum())
+            rows.append(P.counts_rows(st.sections, counts, slice_))
+            if slice_ > 33:
+                stores[slice_] = st
+            else:
+                st.close()
+        if slice_ + 1 in stores:      # the next slice's store is no longer read
+            stores.pop(slice_ + 1).close()
+        rows = np.concatenate(rows)
+        alg = 64 * (nodes + child)
+        r = {"sections": len(secs), "groups": len(groups), "nodes": nodes, "child_nodes_read": child,
+             "plan_build_s": build, "kernel_s": kernel, "superpositions_per_s": nodes / kernel,
+             "algorithmic_GBps": alg / kernel / 1e9, "roofline_frac": alg / kernel / 1e9 / HBM_PEAK_GBPS,
+             "count_kernel_s": count, "count_GBps": 64 * nodes / count / 1e9,
+             "count_roofline_frac": 64 * nodes / count / 1e9 / HBM_PEAK_GBPS,
+             "asymmetric_nodes": asym, "counts_match_production": bool(np.array_equal(rows, want))}
+        log("slice %d: %s" % (slice_, r))
+        out[str(slice_)] = r
+    for st in stores.values():
+        st.close()
+    return out
+
+
 def mid_bench(P, hip, n_boards=4096, seed=2):
     """Batched midsolve: n seeded boards with 18..30 stones in one work-queue launch
     (autotuned tile and workgroups) against one plain pe_midsolve_internal call per board;
@@ -382,6 +482,8 @@ def main():
     ap.add_argument("--mid-boards", type=int, default=4096, help="boards in the batched midsolve measurement")
     ap.add_argument("--no-section30", dest="section30", action="store_false",
                     help="skip the slice-30 section box measurement")
+    ap.add_argument("--no-slices", dest="slices", action="store_false",
+                    help="skip the whole-slice (35, 34, 33) measurement on real data")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU; --gpus %d is not supported" % args.gpus,
@@ -480,6 +582,8 @@ def main():
         del outs, kids
     if not args.no_end and args.section30:
         result["section30"] = section30_bench(P, hip, args.steps)
+    if not args.no_end and args.slices:
+        result["slices"] = slices_bench(P, hip)
     if not args.no_mid:
         result["midsolve"] = mid_bench(P, hip, args.mid_boards)
     if not args.no_serve:
