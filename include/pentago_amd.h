@@ -134,7 +134,8 @@ typedef struct {
  *         once -- the line kernel of pentago/end/compute.cc:219-371 for all four lines
  *         through a block, merged as pentago/end/block_store.cc:173-229);
  * mode 1: four line passes, the later ones OR-accumulating (for comparison).
- * The output section must have at most 34 stones. */
+ * Output sections with 35 stones (one open dimension, no child data) always run as one line
+ * pass. */
 pe_status pe_end_slab_plan_create(const pe_slab* output, int nchildren, const pe_slab* children, int mode,
                                   pe_end_plan** plan);
 /* The same for several output slabs (of one or several sections) in one plan: mode 0 runs
@@ -169,6 +170,16 @@ pe_status pe_end_count_slabs(int n, const pe_slab* slabs, uint64_t (*counts)[4])
  * h_nodes (64 bytes each).  The block store's sample capture, pentago/end/block_store.cc:
  * 190-194, with the index map of pentago/end/check.cc:110-121.  Synchronous. */
 pe_status pe_end_slab_samples(const pe_slab* slab, int n, const uint64_t* boards, void* h_nodes);
+
+/* Supertensor (.pentago) block data <-> nodes, on the device.  A file block holds per node
+ * (black wins, white wins), filter 1 interleaving the two supers bit by bit
+ * (pentago/data/filter.h, applied by pentago/data/supertensor.cc:230-243 and 319-328);
+ * nodes hold (we-win, we-win-or-tie) for the player to move, turn = stones & 1
+ * (orientation as pentago/end/check.cc:149-158).  decode: decompressed file words -> nodes;
+ * encode: nodes -> file words ready for compression.  `nodes` 64-byte entries, in place
+ * allowed, asynchronous.  Compression and file I/O stay on the host. */
+pe_status pe_supertensor_decode(int filter, int turn, const void* d_in, int64_t nodes, void* d_out, void* stream);
+pe_status pe_supertensor_encode(int filter, int turn, const void* d_in, int64_t nodes, void* d_out, void* stream);
 
 /* All standardized sections with `slice` stones that `root` depends on, sorted by sig():
  * slice `slice` of descendent_sections (pentago/end/sections.cc:18-49).  Writes at most cap

@@ -94,6 +94,8 @@ def lib():
             "po_eval_node": (C.c_int, [Section, P(C.c_int), P(Super)]),
             "po_popcounts_over_stabilizers": (C.c_int, [C.c_uint64, P(Super), P(C.c_uint16)]),
             "po_section_counts": (C.c_int, [Section, P(C.c_uint64)]),
+            "po_interleave": (None, [C.c_int64, C.c_void_p]),
+            "po_uninterleave": (None, [C.c_int64, C.c_void_p]),
             "po_brute_value": (C.c_int, [C.c_uint64, C.c_uint64]),
             "po_brute_super": (None, [C.c_uint64, C.c_uint64, P(Super)]),
             "po_half_rmax": (Half, [Half]),
@@ -310,6 +312,19 @@ def count_section(counts):
     r = lib().po_section_counts(section(counts), out)
     assert r == 0, r
     return tuple(out)
+
+
+def interleave(nodes):
+    """(n, 8) uint64 (black, white) nodes -> filter-1 file words (a new array)."""
+    a = np.ascontiguousarray(nodes, dtype=np.uint64).copy()
+    lib().po_interleave(len(a), a.ctypes.data)
+    return a
+
+
+def uninterleave(words):
+    a = np.ascontiguousarray(words, dtype=np.uint64).copy()
+    lib().po_uninterleave(len(a), a.ctypes.data)
+    return a
 
 
 def brute_super(me, them):

@@ -118,6 +118,11 @@ int po_popcounts_over_stabilizers(uint64_t board, const po_super wins[2], uint16
  * (pentago/end/block_store.cc:330-349) summed over the section's blocks. */
 int po_section_counts(po_section s, uint64_t out[3]);
 
+/* ---- supertensor filter (pentago/data/filter.h) ---- */
+/* n nodes of (black, white) supers, in place. */
+void po_interleave(int64_t n, po_super* nodes);
+void po_uninterleave(int64_t n, po_super* nodes);
+
 /* ---- brute force (tests only) ---- */
 /* value in {-1,0,1} of side-to-move `me` vs `them`, next action a stone placement. */
 int po_brute_value(uint64_t me, uint64_t them);

@@ -154,6 +154,8 @@ def lib():
             "pe_end_count_blocks": (C.c_int, [P(Slab), V, V]),
             "pe_end_count_slabs": (C.c_int, [C.c_int, P(Slab), V]),
             "pe_end_slab_samples": (C.c_int, [P(Slab), C.c_int, P(C.c_uint64), V]),
+            "pe_supertensor_decode": (C.c_int, [C.c_int, C.c_int, V, C.c_int64, V, V]),
+            "pe_supertensor_encode": (C.c_int, [C.c_int, C.c_int, V, C.c_int64, V, V]),
             "pe_descendent_sections": (C.c_int, [P(Section), C.c_int, C.c_int, P(Section), P(C.c_int)]),
             "pe_slice_create": (C.c_int, [C.c_int, P(Section), P(V)]),
             "pe_slice_bytes": (C.c_int64, [C.c_int, P(Section)]),

@@ -20,6 +20,8 @@
 namespace pe {
 pe_status count_slab_blocks(const pe_slab& slab, uint64_t* d_counts, hipStream_t stream, std::string* err);
 int64_t slab_blocks(const pe_slab& slab);
+pe_status launch_supertensor_format(bool decode, int filter, int turn, const void* in, int64_t nodes, void* out,
+                                    hipStream_t stream, std::string* err);
 
 namespace {
 
@@ -93,6 +95,20 @@ pe_status pe_end_count_blocks(const pe_slab* slab, uint64_t* d_counts, void* str
 pe_status pe_end_count_slabs(int n, const pe_slab* slabs, uint64_t (*counts)[4]) {
   if (n < 0 || (n && (!slabs || !counts))) return failed(PE_ERR_INVALID, "bad count arguments");
   return count_slabs(n, slabs, counts);
+}
+
+pe_status pe_supertensor_decode(int filter, int turn, const void* d_in, int64_t nodes, void* d_out, void* stream) {
+  std::string err;
+  const pe_status s = launch_supertensor_format(true, filter, turn, d_in, nodes, d_out,
+                                                static_cast<hipStream_t>(stream), &err);
+  return s == PE_OK ? s : failed(s, err);
+}
+
+pe_status pe_supertensor_encode(int filter, int turn, const void* d_in, int64_t nodes, void* d_out, void* stream) {
+  std::string err;
+  const pe_status s = launch_supertensor_format(false, filter, turn, d_in, nodes, d_out,
+                                                static_cast<hipStream_t>(stream), &err);
+  return s == PE_OK ? s : failed(s, err);
 }
 
 pe_status pe_end_slab_samples(const pe_slab* slab, int n, const uint64_t* boards, void* h_nodes) {

@@ -86,6 +86,11 @@ class Buffer:
         check(hip().hipMemset(C.c_void_p(self.ptr), 0, self.nbytes), "hipMemset")
 
 
+def copy_to_device(ptr, array):
+    """Synchronous copy of a contiguous host array to device address ptr."""
+    check(hip().hipMemcpy(C.c_void_p(ptr), C.c_void_p(array.ctypes.data), array.nbytes, H2D), "hipMemcpy H2D")
+
+
 def synchronize():
     check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
 
