@@ -22,7 +22,7 @@ pe_status fused_plan_create(int nout, const pe_slab* outs, int nch, const pe_sla
                             std::string* err);
 pe_status fused_plan_run(const FusedPlan& plan, hipStream_t stream);
 void fused_plan_destroy(FusedPlan* plan);
-pe_status slab_line_passes(const pe_slab& out, int nch, const pe_slab* ch, std::vector<EndPlan>* passes,
+pe_status slab_line_passes(const pe_slab& out, int nch, const pe_slab* ch, std::vector<std::vector<LineHost>>* passes,
                            int64_t* nodes_out, std::string* err);
 pe_status launch_fill_seeded(void* d_nodes, int64_t nodes, uint64_t seed, int64_t first, hipStream_t stream);
 
@@ -261,10 +261,17 @@ pe_status pe_end_slabs_plan_create(int noutputs, const pe_slab* outputs, int nch
     s = fused_plan_create(int(fused.size()), fused.data(), nchildren, children, &plan->fused, &err);
     plan->nodes = plan->fused.output_nodes;
   }
+  // line passes: pass p of every such slab in one launch
+  std::vector<std::vector<LineHost>> groups;
   for (size_t k = 0; k < lined.size() && s == PE_OK; k++) {
     int64_t nodes = 0;
-    s = slab_line_passes(lined[k], nchildren, children, &plan->passes, &nodes, &err);
+    s = slab_line_passes(lined[k], nchildren, children, &groups, &nodes, &err);
     plan->nodes += nodes;
+  }
+  for (size_t p = 0; p < groups.size() && s == PE_OK; p++) {
+    plan->passes.emplace_back();
+    s = plan_create(groups[p], &plan->passes.back());
+    if (s != PE_OK) err = pe_last_error();
   }
   if (s != PE_OK) {
     for (auto& p : plan->passes) plan_destroy(&p);

@@ -443,8 +443,10 @@ void fused_plan_destroy(FusedPlan* plan) {
   plan->nblocks = 0;
 }
 
-// Mode 1: four line passes over the slab, later passes OR-accumulating.
-pe_status slab_line_passes(const pe_slab& out, int nch, const pe_slab* ch, std::vector<EndPlan>* passes,
+// Mode 1: four line passes over the slab, later passes OR-accumulating.  The lines of pass p
+// (the slab's p-th open dimension) are appended to (*passes)[p], so that the passes of many
+// slabs can share one launch each.
+pe_status slab_line_passes(const pe_slab& out, int nch, const pe_slab* ch, std::vector<std::vector<LineHost>>* passes,
                            int64_t* nodes_out, std::string* err) {
   std::vector<LineHost> lines;
   std::map<std::tuple<int, int, int, int>, int> index;
@@ -457,7 +459,7 @@ pe_status slab_line_passes(const pe_slab& out, int nch, const pe_slab* ch, std::
   int64_t nodes = 1;
   for (int a = 0; a < 4; a++) nodes *= std::min(shape[a], 8 * out.hi[a]) - 8 * out.lo[a];
   *nodes_out = nodes;
-  int pass = 0;
+  size_t pass = 0;
   for (int d = 0; d < 4; d++) {
     std::vector<LineHost> group;
     for (auto& h : lines) {
@@ -477,10 +479,8 @@ pe_status slab_line_passes(const pe_slab& out, int nch, const pe_slab* ch, std::
       group.push_back(std::move(g));
     }
     if (group.empty()) continue;
-    EndPlan p;
-    st = plan_create(group, &p);
-    if (st != PE_OK) return st;
-    passes->push_back(p);
+    if (passes->size() <= pass) passes->resize(pass + 1);
+    for (auto& g : group) (*passes)[pass].push_back(std::move(g));
     pass++;
   }
   return PE_OK;

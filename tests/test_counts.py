@@ -142,3 +142,19 @@ def test_descendent_sections_of_a_small_root():
             if all(sum(x) <= 9 for x in c):
                 expect.add(P.section_sig(P.standardize(tuple(map(tuple, c)))[0]))
     assert sorted(P.section_sig(s) for s in got) == sorted(expect)
+
+
+def test_slice_driver_grouping():
+    from pentago_amd import slices as S
+    root = ((0, 0),) * 4
+    s32 = P.descendent_sections(root, 32)
+    kids = set()
+    for s in s32:
+        kids.update(S.child_sections(s))
+    assert sorted(kids) == sorted(P.descendent_sections(root, 33))
+    budget = 150e9
+    gs = S.groups(s32, budget, S.child_sections)
+    assert [s for g in gs for s in g] == s32
+    for g in gs:
+        need = P.slice_bytes(g) + P.slice_bytes(sorted({c for s in g for c in S.child_sections(s)}))
+        assert need <= budget or len(g) == 1

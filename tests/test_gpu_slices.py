@@ -1,8 +1,9 @@
 """Whole slices on the GPU against the production run.
 
-Slices 35, 34 and 33 (every section the empty board reaches, pentago/end/sections.cc:18-49)
+Slices 35, 34, 33 and 32 (every section the empty board reaches, pentago/end/sections.cc:18-49)
 are computed on the device in slice stores -- slice 35 from nothing, 34 from 35, 33 from 34
-in groups that fit next to slice 34 -- and each section's count_block_wins total
+in groups that fit next to slice 34, 32 in groups whose slice-33 children are regenerated
+from slice 34 -- and each section's count_block_wins total
 (pentago/end/block_store.cc:330-349) is compared, converted as pentago/mpi/io.cc:573-581,
 with the reference's data/counts-<slice>.npy rows (the production run's counts, committed
 as fixtures).  The production sparse samples of each slice (pentago/high/high_test.cc:41-58)
@@ -17,6 +18,7 @@ import pytest
 
 import pentago_amd as P
 from pentago_amd import hip
+from pentago_amd import slices as S
 from oracle import oracle as O
 from tests.common import golden
 
@@ -150,26 +152,25 @@ def test_slice_33_matches_production_counts():
     free, _ = hip.mem_info()
     budget = free - 16e9
     assert budget > 20e9, "not enough device memory beside slice 34"
-    groups, cur, used = [], [], 0
-    for sec in secs:
-        b = P.slice_bytes([sec])
-        if cur and used + b > budget:
-            groups.append(cur)
-            cur, used = [], 0
-        cur.append(sec)
-        used += b
-    groups.append(cur)
-    rows, samples = [], 0
-    for g in groups:
-        st = P.SliceStore(g)
-        try:
-            st.compute(children)
-            counts = st.counts()
-            assert int(counts[:, 3].sum()) == 0
-            rows.append(P.counts_rows(st.sections, counts, 33))
-            samples += check_samples(33, lambda sec: st.slab(st.index[sec]) if sec in st.index else None)
-        finally:
-            st.close()
+    samples = [0]
+
+    def visit(st):
+        samples[0] += check_samples(33, lambda sec: st.slab(st.index[sec]) if sec in st.index else None)
+
+    done, counts = S.grouped_counts(secs, budget, children=children, visit=visit)
+    assert int(counts[:, 3].sum()) == 0
+    assert np.array_equal(P.counts_rows(done, counts, 33), golden_rows(33))
+    assert samples[0] == 32
+
+
+def test_slice_32_matches_production_counts():
+    """Slice 32 (1.4 TB dense) in groups, each group's slice-33 children regenerated from
+    the resident slice 34 first."""
+    grandchildren = store(34)
+    secs = P.descendent_sections(EMPTY, 32)
+    free, _ = hip.mem_info()
+    budget = free - 16e9
+    done, counts = S.grouped_counts(secs, budget, grandchildren=grandchildren)
     release(34)
-    assert np.array_equal(np.concatenate(rows), golden_rows(33))
-    assert samples == 32
+    assert int(counts[:, 3].sum()) == 0
+    assert np.array_equal(P.counts_rows(done, counts, 32), golden_rows(32))
