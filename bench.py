@@ -227,7 +227,7 @@ def whole_section_nodes(P, counts):
     return nodes, child
 
 
-def slices_bench(P, hip, budget=None):
+def slices_bench(P, hip, budget=None, slice32=False):
     """Real data: slices 35, 34 and 33 (every section the empty board reaches) computed on
     the device from nothing, slice 33 in groups of sections beside slice 34; kernel time
     from HIP events on the launch stream, plan build time apart, the count kernel timed over
@@ -295,7 +295,7 @@ def slices_bench(P, hip, budget=None):
                 stores[slice_] = st
             else:
                 st.close()
-        if slice_ + 1 in stores:      # the next slice's store is no longer read
+        if slice_ + 1 in stores and not (slice32 and slice_ == 33):   # no longer read
             stores.pop(slice_ + 1).close()
         rows = np.concatenate(rows)
         alg = 64 * (nodes + child)
@@ -307,6 +307,25 @@ def slices_bench(P, hip, budget=None):
              "asymmetric_nodes": asym, "counts_match_production": bool(np.array_equal(rows, want))}
         log("slice %d: %s" % (slice_, r))
         out[str(slice_)] = r
+    if slice32:
+        # slice 32 in groups, each group's slice-33 children regenerated from slice 34;
+        # wall time includes plan builds, the regeneration and the counts
+        from pentago_amd import slices as S
+        secs = P.descendent_sections(empty, 32)
+        want = np.load(os.path.join(golden, "counts-32.npy"), allow_pickle=False)
+        want = want[np.argsort(want[:, 0], kind="stable")]
+        free, _ = hip.mem_info()
+        hip.synchronize()
+        t0 = time.time()
+        done, counts = S.grouped_counts(secs, budget or free - 16e9, grandchildren=stores[34])
+        hip.synchronize()
+        wall = time.time() - t0
+        nodes = sum(whole_section_nodes(P, s)[0] for s in secs)
+        r = {"sections": len(secs), "nodes": nodes, "wall_s": wall, "superpositions_per_s_wall": nodes / wall,
+             "asymmetric_nodes": int(counts[:, 3].sum()),
+             "counts_match_production": bool(np.array_equal(P.counts_rows(done, counts, 32), want))}
+        log("slice 32: %s" % r)
+        out["32"] = r
     for st in stores.values():
         st.close()
     return out
@@ -484,6 +503,8 @@ def main():
                     help="skip the slice-30 section box measurement")
     ap.add_argument("--no-slices", dest="slices", action="store_false",
                     help="skip the whole-slice (35, 34, 33) measurement on real data")
+    ap.add_argument("--slice32", action="store_true",
+                    help="also compute slice 32 in groups (about a minute) and check its counts")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU; --gpus %d is not supported" % args.gpus,
@@ -583,7 +604,7 @@ def main():
     if not args.no_end and args.section30:
         result["section30"] = section30_bench(P, hip, args.steps)
     if not args.no_end and args.slices:
-        result["slices"] = slices_bench(P, hip)
+        result["slices"] = slices_bench(P, hip, slice32=args.slice32)
     if not args.no_mid:
         result["midsolve"] = mid_bench(P, hip, args.mid_boards)
     if not args.no_serve:

@@ -192,6 +192,9 @@ pe_status pe_descendent_sections(const pe_section* root, int slice, int cap, pe_
 typedef struct pe_slice pe_slice;
 /* Allocate device slabs for `n` sections with the same stone count (<= 35). */
 pe_status pe_slice_create(int n, const pe_section* sections, pe_slice** out);
+/* The same with the slabs carved, in order, from caller device memory `arena` of
+ * `arena_bytes` (nothing allocated; the arena must outlive the store). */
+pe_status pe_slice_create_in(int n, const pe_section* sections, void* arena, int64_t arena_bytes, pe_slice** out);
 /* Device bytes the slabs of these sections take (no allocation). */
 int64_t pe_slice_bytes(int n, const pe_section* sections);
 /* Compute every section of `out` from `children` (the store of the next slice holding every
@@ -204,6 +207,21 @@ int pe_slice_sections(const pe_slice* s);
 /* pe_end_count_slabs over every section of the store. */
 pe_status pe_slice_counts(const pe_slice* s, uint64_t (*counts)[4]);
 pe_status pe_slice_destroy(pe_slice* s);
+/* The slice loop for a slice too large to keep whole (pentago/mpi/toplevel.cc:263-421 on one
+ * device): sections[0..n) of one slice are computed in groups, in order, each group's slabs
+ * (and its regenerated children) within `budget` bytes.  Children come from `children`, the
+ * whole store of the next slice, or -- when it is NULL -- are regenerated per group from
+ * `grandchildren`, the whole store of the slice after it.  counts[k] = pe_slice_counts of
+ * sections[k]; visit(ctx, group), if not NULL, sees every computed group before it is freed
+ * (for samples).  *groups (may be NULL) receives the number of groups. */
+/* The groups pe_slice_grouped_counts uses (host only): group g is sections
+ * [ends[g-1], ends[g]) with ends[-1] = 0; ends needs room for n entries. */
+pe_status pe_slice_plan_groups(int n, const pe_section* sections, int64_t budget, int regenerate, int* ends,
+                               int* ngroups);
+pe_status pe_slice_grouped_counts(int n, const pe_section* sections, int64_t budget, const pe_slice* children,
+                                  const pe_slice* grandchildren, uint64_t (*counts)[4],
+                                  void (*visit)(void* ctx, const pe_slice* group), void* ctx, int* groups,
+                                  void* stream);
 
 /* ---------------- midsolve ---------------- */
 

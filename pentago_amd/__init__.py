@@ -158,12 +158,16 @@ def lib():
             "pe_supertensor_encode": (C.c_int, [C.c_int, C.c_int, V, C.c_int64, V, V]),
             "pe_descendent_sections": (C.c_int, [P(Section), C.c_int, C.c_int, P(Section), P(C.c_int)]),
             "pe_slice_create": (C.c_int, [C.c_int, P(Section), P(V)]),
+            "pe_slice_create_in": (C.c_int, [C.c_int, P(Section), V, C.c_int64, P(V)]),
             "pe_slice_bytes": (C.c_int64, [C.c_int, P(Section)]),
             "pe_slice_compute": (C.c_int, [V, V, V]),
             "pe_slice_slab": (C.c_int, [V, C.c_int, P(Slab)]),
             "pe_slice_sections": (C.c_int, [V]),
             "pe_slice_counts": (C.c_int, [V, V]),
             "pe_slice_destroy": (C.c_int, [V]),
+            "pe_slice_plan_groups": (C.c_int, [C.c_int, P(Section), C.c_int64, C.c_int, P(C.c_int), P(C.c_int)]),
+            "pe_slice_grouped_counts": (C.c_int, [C.c_int, P(Section), C.c_int64, V, V, V,
+                                                  C.CFUNCTYPE(None, V, V), V, P(C.c_int), V]),
             "pe_midsolve_workspace_size": (C.c_int64, [C.c_int]),
             "pe_midsolve_internal": (C.c_int, [P(HighBoard), P(HalfSupers)]),
             "pe_midsolve": (C.c_int, [P(HighBoard), P(MidValue), P(C.c_int)]),

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <unordered_map>
 #include <tuple>
 #include <vector>
 
@@ -255,6 +256,13 @@ static pe_status slab_lines(const pe_slab& out, int nch, const pe_slab* ch, std:
   if (s.sum() > 35) { *err = "slab plans need sections with at most 35 stones"; return PE_ERR_INVALID; }
   for (int a = 0; a < 4; a++)
     if (out.lo[a] < 0 || out.hi[a] <= out.lo[a] || 8 * (out.hi[a] - 1) >= shape[a]) { *err = "output box outside the section"; return PE_ERR_INVALID; }
+  // child slabs by section
+  std::unordered_map<uint64_t, std::vector<int>> by_section;
+  for (int c = 0; c < nch; c++) {
+    Section cs;
+    std::memcpy(cs.c, ch[c].section.counts, 8);
+    by_section[cs.sig()].push_back(c);
+  }
   for (int d = 0; d < 4; d++) {
     if (s.c[d][0] + s.c[d][1] >= 9) continue;
     int o[3], k = 0;
@@ -277,8 +285,12 @@ static pe_status slab_lines(const pe_slab& out, int nch, const pe_slab* ch, std:
           }
           const int cd = h.info.child_dimension;
           const pe_slab* found = nullptr;
-          for (int c = 0; c < nch && !found; c++) {
-            if (!same_section(ch[c].section, h.info.standard_child)) continue;
+          Section want;
+          std::memcpy(want.c, h.info.standard_child.counts, 8);
+          const auto it = by_section.find(want.sig());
+          const std::vector<int> none;
+          for (int c : it == by_section.end() ? none : it->second) {
+            if (found) break;
             int cb[4];
             for (int a = 0; a < 4; a++) cb[a] = h.info.first_child_block[a];
             bool ok = true;

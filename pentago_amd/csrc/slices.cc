@@ -35,6 +35,38 @@ bool section_of(const pe_section& p, Section* s, int shape[4]) {
   return s->valid_counts() && s->shape(shape);
 }
 
+Section section_from_sig(uint64_t sig) {
+  Section c;
+  for (int q = 0; q < 4; q++) {
+    c.c[q][0] = uint8_t(sig >> (16 * q));
+    c.c[q][1] = uint8_t(sig >> (16 * q + 8));
+  }
+  return c;
+}
+
+// Standardized child sections of sections [begin, end), sorted by sig.
+std::vector<pe_section> group_kids(const pe_section* sections, int begin, int end) {
+  std::set<uint64_t> sigs;
+  for (int k = begin; k < end; k++) {
+    Section s;
+    std::memcpy(s.c, sections[k].counts, 8);
+    for (int q = 0; q < 4; q++)
+      if (s.c[q][0] + s.c[q][1] < 9) {
+        Section t;
+        s.child(q).standardize(&t);
+        sigs.insert(t.sig());
+      }
+  }
+  std::vector<pe_section> out;
+  for (uint64_t sig : sigs) {
+    const Section c = section_from_sig(sig);
+    pe_section p;
+    std::memcpy(p.counts, c.c, 8);
+    out.push_back(p);
+  }
+  return out;
+}
+
 pe_slab whole_slab(const pe_section& sec, const int shape[4], void* data) {
   pe_slab slab;
   slab.section = sec;
@@ -188,10 +220,11 @@ int64_t pe_slice_bytes(int n, const pe_section* sections) {
   return bytes;
 }
 
-pe_status pe_slice_create(int n, const pe_section* sections, pe_slice** out) {
-  if (n <= 0 || !sections || !out) return failed(PE_ERR_INVALID, "bad slice arguments");
+pe_status pe_slice_create_in(int n, const pe_section* sections, void* arena, int64_t arena_bytes, pe_slice** out) {
+  if (n <= 0 || !sections || !out || (arena && arena_bytes < 0)) return failed(PE_ERR_INVALID, "bad slice arguments");
   auto* st = new pe_slice();
   (void)hipGetDevice(&st->device);
+  int64_t used = 0;
   for (int k = 0; k < n; k++) {
     Section s, t;
     int shape[4];
@@ -207,16 +240,30 @@ pe_status pe_slice_create(int n, const pe_section* sections, pe_slice** out) {
     st->slice = s.sum();
     int64_t blocks = 1;
     for (int a = 0; a < 4; a++) blocks *= (shape[a] + 7) / 8;
+    const int64_t bytes = blocks * 4096 * 64;
     void* d = nullptr;
-    if (hipMalloc(&d, size_t(blocks) * 4096 * 64) != hipSuccess) {
-      pe_slice_destroy(st);
-      return failed(PE_ERR_CAPACITY, "hipMalloc of section " + std::to_string(k) + " failed");
+    if (arena) {   // carve from the caller's memory (no allocation)
+      if (used + bytes > arena_bytes) {
+        pe_slice_destroy(st);
+        return failed(PE_ERR_CAPACITY, "sections do not fit the arena");
+      }
+      d = static_cast<uint8_t*>(arena) + used;
+      used += bytes;
+    } else {
+      if (hipMalloc(&d, size_t(bytes)) != hipSuccess) {
+        pe_slice_destroy(st);
+        return failed(PE_ERR_CAPACITY, "hipMalloc of section " + std::to_string(k) + " failed");
+      }
+      st->data.push_back(d);
     }
-    st->data.push_back(d);
     st->slabs.push_back(whole_slab(sections[k], shape, d));
   }
   *out = st;
   return PE_OK;
+}
+
+pe_status pe_slice_create(int n, const pe_section* sections, pe_slice** out) {
+  return pe_slice_create_in(n, sections, nullptr, 0, out);
 }
 
 pe_status pe_slice_compute(pe_slice* out, const pe_slice* children, void* stream) {
@@ -246,6 +293,96 @@ int pe_slice_sections(const pe_slice* s) { return s ? int(s->slabs.size()) : -1;
 pe_status pe_slice_counts(const pe_slice* s, uint64_t (*counts)[4]) {
   if (!s || !counts) return failed(PE_ERR_INVALID, "null argument");
   return count_slabs(int(s->slabs.size()), s->slabs.data(), counts);
+}
+
+pe_status pe_slice_plan_groups(int n, const pe_section* sections, int64_t budget, int regenerate, int* ends,
+                               int* ngroups) {
+  if (n <= 0 || !sections || !ends || !ngroups || budget <= 0) return failed(PE_ERR_INVALID, "bad group arguments");
+  auto bytes_of = [](const Section& s) {
+    pe_section p;
+    std::memcpy(p.counts, s.c, 8);
+    return pe_slice_bytes(1, &p);
+  };
+  int count = 0;
+  int64_t used = 0;
+  std::set<uint64_t> kids;
+  for (int k = 0; k < n; k++) {
+    Section s;
+    int shape[4];
+    if (!section_of(sections[k], &s, shape)) return failed(PE_ERR_INVALID, "invalid section " + std::to_string(k));
+    std::set<uint64_t> all, fresh;
+    if (regenerate)
+      for (int q = 0; q < 4; q++)
+        if (s.c[q][0] + s.c[q][1] < 9) {
+          Section t;
+          s.child(q).standardize(&t);
+          all.insert(t.sig());
+        }
+    auto kid_bytes = [&](const std::set<uint64_t>& sigs) {
+      int64_t b = 0;
+      for (uint64_t sig : sigs) b += bytes_of(section_from_sig(sig));
+      return b;
+    };
+    for (uint64_t sig : all)
+      if (!kids.count(sig)) fresh.insert(sig);
+    const int64_t need = bytes_of(s);
+    if (k > 0 && used + need + kid_bytes(fresh) > budget) {
+      ends[count++] = k;
+      used = 0;
+      kids.clear();
+      fresh = all;
+    }
+    used += need + kid_bytes(fresh);
+    kids.insert(fresh.begin(), fresh.end());
+  }
+  ends[count++] = n;
+  *ngroups = count;
+  return PE_OK;
+}
+
+pe_status pe_slice_grouped_counts(int n, const pe_section* sections, int64_t budget, const pe_slice* children,
+                                  const pe_slice* grandchildren, uint64_t (*counts)[4],
+                                  void (*visit)(void* ctx, const pe_slice* group), void* ctx, int* ngroups,
+                                  void* stream) {
+  if (!counts) return failed(PE_ERR_INVALID, "null counts");
+  if (!children && !grandchildren) return failed(PE_ERR_INVALID, "need the next slice's store or the one after it");
+  const bool regen = !children;
+  std::vector<int> ends(std::max(n, 1));
+  int ng = 0;
+  pe_status st = pe_slice_plan_groups(n, sections, budget, regen, ends.data(), &ng);
+  if (st != PE_OK) return st;
+  if (ngroups) *ngroups = ng;
+  // one arena for every group (its outputs, then its regenerated children): device memory
+  // is allocated once, not per section and group
+  int64_t arena_bytes = 0;
+  for (int g = 0, begin = 0; g < ng; begin = ends[g++]) {
+    int64_t b = pe_slice_bytes(ends[g] - begin, sections + begin);
+    if (regen) b += pe_slice_bytes(int(group_kids(sections, begin, ends[g]).size()),
+                                   group_kids(sections, begin, ends[g]).data());
+    arena_bytes = std::max(arena_bytes, b);
+  }
+  void* arena = nullptr;
+  if (hipMalloc(&arena, size_t(arena_bytes)) != hipSuccess) return failed(PE_ERR_CAPACITY, "hipMalloc(group arena) failed");
+  for (int g = 0, begin = 0; g < ng; begin = ends[g++]) {
+    pe_slice* kids = nullptr;
+    const int64_t out_bytes = pe_slice_bytes(ends[g] - begin, sections + begin);
+    if (regen) {
+      const std::vector<pe_section> ks = group_kids(sections, begin, ends[g]);
+      st = pe_slice_create_in(int(ks.size()), ks.data(), static_cast<uint8_t*>(arena) + out_bytes,
+                              arena_bytes - out_bytes, &kids);
+      if (st == PE_OK) st = pe_slice_compute(kids, grandchildren, stream);
+    }
+    pe_slice* out = nullptr;
+    if (st == PE_OK) st = pe_slice_create_in(ends[g] - begin, sections + begin, arena, out_bytes, &out);
+    if (st == PE_OK) st = pe_slice_compute(out, regen ? kids : children, stream);
+    if (st == PE_OK) st = pe_slice_counts(out, counts + begin);
+    if (st == PE_OK && visit) visit(ctx, out);
+    pe_slice_destroy(out);
+    pe_slice_destroy(kids);
+    if (st != PE_OK) break;
+  }
+  (void)hipFree(arena);
+  return st;
 }
 
 pe_status pe_slice_destroy(pe_slice* s) {

@@ -2,14 +2,16 @@
 without MPI, compression or files).
 
 Slices are computed from 35 down.  A slice that fits beside the next one is kept whole
-(SliceStore); one that does not is computed in groups of sections, each group counted (and
-sampled) and freed before the next.  Two slices below the last resident one, a group's child
-sections are regenerated from the resident slice first (they are needed only while the
-group runs).
+(SliceStore); one that does not is computed in groups of sections by the native loop
+pe_slice_grouped_counts, each group counted (and sampled) and freed before the next.  Two
+slices below the last resident one, a group's child sections are regenerated from the
+resident slice first (they are needed only while the group runs).
 """
+import ctypes as C
+
 import numpy as np
 
-from . import SliceStore, slice_bytes, standardize
+from . import Section, Slab, _check, lib, standardize
 
 
 def child_sections(section):
@@ -25,58 +27,67 @@ def child_sections(section):
     return sorted(out)
 
 
-def groups(sections, budget, kids_of=None):
-    """Split sections (in order) into groups whose slabs fit `budget` bytes, counting also
-    the slabs of the union of kids_of(section) over the group when given."""
-    size = {}
-
-    def nbytes(sec):
-        if sec not in size:
-            size[sec] = slice_bytes([sec])
-        return size[sec]
-
-    out, cur, kids, used = [], [], set(), 0
-    for s in sections:
-        new = set(kids_of(s)) - kids if kids_of else set()
-        need = nbytes(s) + sum(nbytes(k) for k in new)
-        if cur and used + need > budget:
-            out.append(cur)
-            cur, kids, used = [], set(), 0
-            new = set(kids_of(s)) if kids_of else set()
-            need = nbytes(s) + sum(nbytes(k) for k in new)
-        cur.append(s)
-        kids |= new
-        used += need
-    if cur:
-        out.append(cur)
+def plan_groups(sections, budget, regenerate):
+    """The groups grouped_counts computes (host only): list of section lists, in order."""
+    arr = (Section * len(sections))(*[Section.of(c) for c in sections])
+    ends = (C.c_int * len(sections))()
+    n = C.c_int()
+    _check(lib().pe_slice_plan_groups(len(sections), arr, int(budget), int(bool(regenerate)), ends, C.byref(n)),
+           "pe_slice_plan_groups")
+    out, begin = [], 0
+    for g in range(n.value):
+        out.append(list(sections[begin:ends[g]]))
+        begin = ends[g]
     return out
 
 
+class GroupView:
+    """A computed group inside grouped_counts (valid during the visit only)."""
+
+    def __init__(self, handle):
+        self.handle = handle
+        n = lib().pe_slice_sections(handle)
+        self.slabs = []
+        for k in range(n):
+            slab = Slab()
+            _check(lib().pe_slice_slab(handle, k, C.byref(slab)), "pe_slice_slab")
+            self.slabs.append(slab)
+        self.sections = [s.section.counts for s in self.slabs]
+        self.index = {s: k for k, s in enumerate(self.sections)}
+
+    def slab(self, k):
+        return self.slabs[k]
+
+
+VISIT = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p)
+
+
 def grouped_counts(sections, budget, children=None, grandchildren=None, visit=None, stream=None):
-    """Counts of every section, computed in groups that fit `budget` bytes.
+    """Counts of every section of one slice, computed in groups within `budget` bytes by the
+    native slice loop (pe_slice_grouped_counts).
 
     children: the whole store of the next slice, or None with grandchildren: the whole store
     of the slice after it, from which each group's child sections are regenerated.
-    visit(store) is called with each computed group before it is freed.  Returns
+    visit(group) sees each computed group (a GroupView) before it is freed.  Returns
     (sections in order, (n, 4) counts)."""
     if children is None and grandchildren is None:
         raise ValueError("need the next slice's store or the one after it")
-    regen = children is None
-    done, counts = [], []
-    for g in groups(sections, budget, child_sections if regen else None):
-        kids = children
-        if regen:
-            kids = SliceStore(sorted({c for s in g for c in child_sections(s)}))
-            kids.compute(grandchildren, stream)
-        st = SliceStore(g)
+    sections = [tuple(tuple(x) for x in s) for s in sections]
+    arr = (Section * len(sections))(*[Section.of(c) for c in sections])
+    counts = np.zeros((len(sections), 4), dtype=np.uint64)
+    errors = []
+
+    def cb(ctx, handle):
         try:
-            st.compute(kids, stream)
-            counts.append(st.counts())
-            done.extend(st.sections)
-            if visit:
-                visit(st)
-        finally:
-            st.close()
-            if regen:
-                kids.close()
-    return done, np.concatenate(counts) if counts else np.zeros((0, 4), dtype=np.uint64)
+            visit(GroupView(handle))
+        except Exception as e:   # re-raised after the native loop returns
+            errors.append(e)
+
+    fn = VISIT(cb) if visit else VISIT()
+    ng = C.c_int()
+    _check(lib().pe_slice_grouped_counts(len(sections), arr, int(budget), children.handle if children else None,
+                                         grandchildren.handle if grandchildren else None, counts.ctypes.data,
+                                         fn, None, C.byref(ng), stream), "pe_slice_grouped_counts")
+    if errors:
+        raise errors[0]
+    return sections, counts

@@ -153,8 +153,16 @@ def test_slice_driver_grouping():
         kids.update(S.child_sections(s))
     assert sorted(kids) == sorted(P.descendent_sections(root, 33))
     budget = 150e9
-    gs = S.groups(s32, budget, S.child_sections)
-    assert [s for g in gs for s in g] == s32
+    gs = S.plan_groups(s32, budget, True)
+    assert [s for g in gs for s in g] == s32 and len(gs) > 1
     for g in gs:
         need = P.slice_bytes(g) + P.slice_bytes(sorted({c for s in g for c in S.child_sections(s)}))
         assert need <= budget or len(g) == 1
+        # greedy: the next group's first section would not have fit
+    for g, h in zip(gs, gs[1:]):
+        more = g + h[:1]
+        assert P.slice_bytes(more) + P.slice_bytes(sorted({c for s in more for c in S.child_sections(s)})) > budget
+    # without regeneration only the sections' own slabs count
+    gs = S.plan_groups(s32, budget, False)
+    assert all(P.slice_bytes(g) <= budget for g in gs)
+    assert all(P.slice_bytes(g + h[:1]) > budget for g, h in zip(gs, gs[1:]))
