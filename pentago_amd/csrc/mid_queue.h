@@ -53,6 +53,7 @@ struct MidPlan {
   std::vector<MidInfo> infos;
   hipStream_t alloc_stream = nullptr;  // stream-ordered allocation (serving) or null
   uint8_t* mem = nullptr;              // layout.bytes
+  bool owns_mem = true;                // false: caller memory, not freed by the plan
   const MidBoardDev* small_dev = nullptr;
   const MidBoardDev* big_dev = nullptr;
   const int64_t* prefix = nullptr;     // small boards' prep prefix, then [0, total] per big board
@@ -67,9 +68,15 @@ struct MidPlan {
   int64_t launches = 0;                // kernel launches per run
 };
 
-// alloc_stream null: synchronous hipMalloc/hipMemcpy; else stream-ordered on it.
+// alloc_stream null: synchronous hipMalloc/hipMemcpy; else stream-ordered on it.  With
+// pinned host memory `upload` of at least the layout's o_res bytes (which must stay untouched
+// until alloc_stream has passed the plan's first run), the descriptor upload is queued
+// without waiting; otherwise the call waits for the upload.
+// With `memory` of at least the layout's bytes the plan lives there (nothing is allocated,
+// the caller keeps the memory until the plan is destroyed and its runs have finished).
 pe_status mid_plan_create(const std::vector<MidInfo>& infos, const MidConfig& cfg, MidPlan* plan,
-                          hipStream_t alloc_stream = nullptr);
+                          hipStream_t alloc_stream = nullptr, void* upload = nullptr, size_t upload_bytes = 0,
+                          void* memory = nullptr, size_t memory_bytes = 0);
 pe_status mid_plan_run(MidPlan& plan, hipStream_t stream);
 // Queue the copy of all results to host memory on `stream`; after the stream has been
 // synchronized, mid_plan_zero_tail clears the entries past each board's spots.

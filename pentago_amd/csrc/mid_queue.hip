@@ -110,7 +110,8 @@ const char* mid_layout(const std::vector<MidInfo>& infos, const MidConfig& cfg, 
 }
 
 pe_status mid_plan_create(const std::vector<MidInfo>& infos, const MidConfig& cfg, MidPlan* plan,
-                          hipStream_t alloc_stream) {
+                          hipStream_t alloc_stream, void* upload, size_t upload_bytes, void* memory,
+                          size_t memory_bytes) {
   DevTables T;
   std::string err;
   if (device_tables(&T, &err) != PE_OK) { set_error(err); return PE_ERR_DEVICE; }
@@ -126,7 +127,9 @@ pe_status mid_plan_create(const std::vector<MidInfo>& infos, const MidConfig& cf
   plan->alloc_stream = alloc_stream;
   PE_HIP(hipGetDevice(&plan->device), "hipGetDevice");
   void* mem = nullptr;
-  if (alloc_stream) PE_HIP(hipMallocAsync(&mem, L.bytes, alloc_stream), "hipMallocAsync(mid batch)");
+  plan->owns_mem = !(memory && L.bytes <= memory_bytes);
+  if (!plan->owns_mem) mem = memory;
+  else if (alloc_stream) PE_HIP(hipMallocAsync(&mem, L.bytes, alloc_stream), "hipMallocAsync(mid batch)");
   else PE_HIP(hipMalloc(&mem, L.bytes), "hipMalloc(mid batch)");
   plan->mem = static_cast<uint8_t*>(mem);
   uint8_t* base = plan->mem;
@@ -165,7 +168,12 @@ pe_status mid_plan_create(const std::vector<MidInfo>& infos, const MidConfig& cf
     p[1] = s.set0 + s.wins1 + s.set1p;
     plan->big_prep_total[k] = p[1];
   }
-  if (alloc_stream) {
+  if (alloc_stream && upload && blob.size() <= upload_bytes) {
+    // pinned staging owned by the caller: the copy is queued, nothing waits
+    std::memcpy(upload, blob.data(), blob.size());
+    PE_HIP(hipMemcpyAsync(base, upload, blob.size(), hipMemcpyHostToDevice, alloc_stream),
+           "hipMemcpyAsync(mid batch)");
+  } else if (alloc_stream) {
     PE_HIP(hipMemcpyAsync(base, blob.data(), blob.size(), hipMemcpyHostToDevice, alloc_stream),
            "hipMemcpyAsync(mid batch)");
     PE_HIP(hipStreamSynchronize(alloc_stream), "mid batch upload");   // blob is a local
@@ -307,7 +315,7 @@ void mid_plan_destroy(MidPlan* plan) {
     (void)hipStreamDestroy(s);
   }
   for (hipEvent_t e : plan->events) (void)hipEventDestroy(e);
-  if (plan->mem) {
+  if (plan->mem && plan->owns_mem) {
     if (plan->alloc_stream) (void)hipFreeAsync(plan->mem, plan->alloc_stream);
     else (void)hipFree(plan->mem);
   }

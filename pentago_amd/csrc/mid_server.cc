@@ -7,8 +7,12 @@
 // device memory (tables, workspace, descriptors) fits in the part of the HBM budget not held
 // by batches in flight (a request that does not fit waits; later, smaller requests may pass
 // it until it has waited kHeadOfLine, after which admission stops until it fits), and runs
-// the batch on that stream as a batched plan.  Batches on different streams overlap on the
-// device.  Each request's state stays in device memory from admission to retirement.
+// the batch on that stream as a batched plan.  A board with more empty cells than the
+// small-board limit is a batch of its own and such batches hold at most all streams but
+// one, so the small boards' batches (one persistent kernel each, about a millisecond) are
+// never queued behind a large board's level launches.  Batches on different streams
+// overlap on the device.  Each request's state stays in device memory from admission to
+// retirement.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -46,6 +50,7 @@ struct Batch {
   std::vector<int64_t> tickets;
   MidPlan plan;
   int64_t bytes = 0;
+  bool large = false;   // one board with more than small_spots empty cells
 };
 
 }  // namespace
@@ -58,6 +63,10 @@ struct MidServer {
   std::vector<hipStream_t> streams;
   std::vector<hipEvent_t> events;
   std::vector<pe_halfsupers*> staging;   // pinned, max_batch * 19 per stream
+  std::vector<void*> upload;             // pinned descriptor staging per stream
+  std::vector<size_t> upload_cap;
+  std::vector<void*> arena;              // device memory per stream, grown to the largest batch
+  std::vector<size_t> arena_cap;
   std::vector<char> stream_busy;
   std::vector<Batch> batch_of_stream;
 
@@ -105,7 +114,50 @@ struct MidServer {
       std::lock_guard<std::mutex> lock(mu);
       for (int64_t t : b.tickets) infos.push_back(requests[t].info);
     }
-    pe_status st = mid_plan_create(infos, cfg, &b.plan, s);
+    // descriptors go up through this stream's pinned staging, so creating the plan never
+    // waits for the device
+    MidLayout L;
+    if (const char* e = mid_layout(infos, cfg, &L)) {
+      set_error(e);
+      return PE_ERR_CAPACITY;
+    }
+    if (L.o_res > upload_cap[size_t(k)]) {
+      if (upload[size_t(k)]) (void)hipHostFree(upload[size_t(k)]);
+      upload[size_t(k)] = nullptr;
+      upload_cap[size_t(k)] = 0;
+      const size_t cap = std::max<size_t>(L.o_res, size_t(1) << 20);
+      PE_HIP(hipHostMalloc(&upload[size_t(k)], cap, hipHostMallocDefault), "hipHostMalloc(server upload)");
+      upload_cap[size_t(k)] = cap;
+    }
+    // each stream keeps one device arena (batches on a stream run one after another), so a
+    // batch allocates nothing once the arena has grown to the largest batch it runs;
+    // per-batch stream-ordered allocations made concurrent streams wait on each other
+    // The arenas together stay within the budget: idle streams give theirs up when a
+    // stream needs to grow, and a batch that still does not fit falls back to a
+    // stream-ordered allocation of its own.
+    if (L.bytes > arena_cap[size_t(k)]) {
+      const size_t cap = std::max<size_t>(L.bytes + L.bytes / 4, size_t(64) << 20);
+      auto held = [&] {
+        size_t h = 0;
+        for (size_t j = 0; j < arena.size(); j++) h += j == size_t(k) ? 0 : arena_cap[j];
+        return h;
+      };
+      for (size_t j = 0; j < arena.size() && held() + cap > size_t(budget); j++)
+        if (j != size_t(k) && !stream_busy[j] && arena[j]) {
+          (void)hipFree(arena[j]);
+          arena[j] = nullptr;
+          arena_cap[j] = 0;
+        }
+      if (held() + cap <= size_t(budget)) {
+        if (arena[size_t(k)]) (void)hipFree(arena[size_t(k)]);
+        arena[size_t(k)] = nullptr;
+        arena_cap[size_t(k)] = 0;
+        PE_HIP(hipMalloc(&arena[size_t(k)], cap), "hipMalloc(server arena)");
+        arena_cap[size_t(k)] = cap;
+      }
+    }
+    pe_status st = mid_plan_create(infos, cfg, &b.plan, s, upload[size_t(k)], upload_cap[size_t(k)], arena[size_t(k)],
+                                   arena_cap[size_t(k)]);
     if (st != PE_OK) return st;
     st = mid_plan_run(b.plan, s);
     if (st != PE_OK) return st;
@@ -143,21 +195,53 @@ struct MidServer {
         {
           std::lock_guard<std::mutex> lock(mu);
           const auto now = Clock::now();
+          // A large board (more empty cells than small_spots: full-device level launches,
+          // milliseconds) runs alone, on at most all streams but one; small boards batch
+          // together on the others, so they never wait behind a large board's levels.
+          int large_busy = 0;
+          for (size_t j = 0; j < streams.size(); j++) large_busy += stream_busy[j] && batch_of_stream[j].large;
+          const bool large_ok = streams.size() == 1 || large_busy + 1 < int(streams.size());
+          // bound >= the batch's exact bytes (every small board counted with a full workspace
+          // slot, descriptors with their alignment), updated in O(1) per request; the exact
+          // layout is computed only when the bound does not fit
+          int64_t tables = 0, max_work = 0, max_slot = 0, nsmall = 0;
           for (auto it = pending.begin(); it != pending.end() && int32_t(b.tickets.size()) < max_batch;) {
             Request& r = requests[*it];
+            const bool large = r.info.spots > cfg.small_spots;
+            if (large && (!b.tickets.empty() || !large_ok)) {
+              ++it;
+              continue;
+            }
             infos.push_back(r.info);
-            const int64_t bytes = batch_bytes(infos);
-            if (in_use + bytes > budget) {
+            int64_t t2 = tables, w2 = max_work, s2 = max_slot, n2 = nsmall;
+            if (large) {
+              s2 = std::max<int64_t>(s2, int64_t(mid_table_bytes(r.info) + mid_work_bytes(r.info)));
+            } else {
+              t2 += int64_t(mid_table_bytes(r.info));
+              w2 = std::max<int64_t>(w2, int64_t(mid_work_bytes(r.info)));
+              n2++;
+            }
+            // per board: a descriptor (MidInfo + 6 pointers, padded), 3 prefix entries and
+            // 19 results; 256-byte alignment of the 4 blob parts and the queue head
+            const int64_t per_board = int64_t(sizeof(MidInfo) + 8 * sizeof(void*) + 3 * sizeof(int64_t) +
+                                              19 * sizeof(pe_halfsupers));
+            const int64_t bound = t2 + n2 * w2 + s2 + int64_t(infos.size()) * per_board + 5 * 256 + 4096;
+            if (in_use + bound > budget && in_use + batch_bytes(infos) > budget) {
               infos.pop_back();
               // the oldest waiting request blocks further admission once it has waited long
               if (it == pending.begin() && now - r.submitted > kHeadOfLine) break;
               ++it;
               continue;
             }
-            b.bytes = bytes;
+            tables = t2, max_work = w2, max_slot = s2, nsmall = n2;
             b.tickets.push_back(*it);
             it = pending.erase(it);
+            if (large) {
+              b.large = true;
+              break;
+            }
           }
+          if (!b.tickets.empty()) b.bytes = batch_bytes(infos);
           if (!b.tickets.empty()) {
             in_use += b.bytes;
             peak = std::max(peak, in_use);
@@ -210,12 +294,7 @@ pe_status mid_server_create(const pe_mid_server_config* c, MidServer** out) {
   S->cfg.workgroups = std::max(S->cfg.workgroups, 1);
   S->cfg.streams = 0;   // a batch runs entirely on its server stream
   S->cfg.graph = false;
-  // keep freed stream-ordered memory cached for the next batches
-  hipMemPool_t pool = nullptr;
-  if (hipDeviceGetDefaultMemPool(&pool, S->device) == hipSuccess) {
-    uint64_t threshold = uint64_t(budget);
-    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
-  }
+
   for (int k = 0; k < nstreams; k++) {
     hipStream_t s = nullptr;
     hipEvent_t e = nullptr;
@@ -227,6 +306,10 @@ pe_status mid_server_create(const pe_mid_server_config* c, MidServer** out) {
     S->streams.push_back(s);
     S->events.push_back(e);
     S->staging.push_back(static_cast<pe_halfsupers*>(h));
+    S->upload.push_back(nullptr);
+    S->upload_cap.push_back(0);
+    S->arena.push_back(nullptr);
+    S->arena_cap.push_back(0);
   }
   S->stream_busy.assign(size_t(nstreams), 0);
   S->batch_of_stream.resize(size_t(nstreams));
@@ -306,6 +389,8 @@ void mid_server_destroy(MidServer* S) {
     (void)hipStreamDestroy(S->streams[k]);
     (void)hipEventDestroy(S->events[k]);
     (void)hipHostFree(S->staging[k]);
+    if (S->upload[k]) (void)hipHostFree(S->upload[k]);
+    if (S->arena[k]) (void)hipFree(S->arena[k]);
   }
   delete S;
 }
