@@ -13,7 +13,7 @@ passes per box, OR-merging into the output.  Inputs are on the device before tim
 
 Prints one JSON line.  Metric: superpositions/s = output nodes per second (the reference's
 output nodes/second, pentago/mpi/toplevel.cc:83-94).  The same line carries a box of the
-slice-30 section, whole slices 35, 34 and 33 on real data (computed from nothing, counts
+slice-30 section, whole slices 35, 34, 33 and 32 on real data (computed from nothing, counts
 checked against the production run's), the batched midsolve (boards/s against plain
 per-board launches) and serving (p50/p99 latency at Poisson loads) measurements unless
 --no-section30 / --no-slices / --no-mid / --no-serve.
@@ -503,8 +503,8 @@ def main():
                     help="skip the slice-30 section box measurement")
     ap.add_argument("--no-slices", dest="slices", action="store_false",
                     help="skip the whole-slice (35, 34, 33) measurement on real data")
-    ap.add_argument("--slice32", action="store_true",
-                    help="also compute slice 32 in groups (about a minute) and check its counts")
+    ap.add_argument("--no-slice32", dest="slice32", action="store_false",
+                    help="skip slice 32 (computed in groups with regenerated children, ~13 s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU; --gpus %d is not supported" % args.gpus,
