@@ -186,6 +186,39 @@ def copy_probe(nbytes):
 SECTION30 = ((3, 4), (4, 4), (4, 4), (4, 3))   # standard form of (4,4)(4,3)(3,4)(4,4)
 
 
+def line_kernel_bench(P, hip, steps, k=8):
+    """The plain line form beside the fused kernel: the same k^4-block box of section
+    (3,3)^4 computed as four line passes (end_lines_kernel, one line per workgroup row, the
+    schedule_compute_line drop-in) and as one fused launch; outputs compared."""
+    specs = [(SECTION, [4] * 4, [4 + k] * 4)]
+    out = {"workload": "box [4,%d)^4 of section 24-(3,3)^4" % (4 + k)}
+    res = {}
+    for mode, name in ((1, "line_passes"), (0, "fused")):
+        outs, kids, plan, n_out, child_nodes = build_boxes(P, hip, specs, mode)
+        stream = hip.Stream()
+        plan.run(stream.ptr)
+        stream.synchronize()
+        ev0, ev1 = hip.Event(), hip.Event()
+        ms = []
+        for _ in range(max(steps, 1)):
+            ev0.record(stream)
+            plan.run(stream.ptr)
+            ev1.record(stream)
+            ev1.synchronize()
+            ms.append(ev0.elapsed_ms(ev1))
+        plan.close()
+        t = min(ms)
+        buf = outs[0][3]
+        res[name] = buf.download(np.zeros(buf.nbytes // 8, dtype=np.uint64))
+        out[name] = {"ms": t, "superpositions_per_s": n_out / (t / 1e3)}
+        del outs, kids
+    # same seeds in both builds: identical child data, so outputs must agree bit for bit
+    out["outputs_equal"] = bool(np.array_equal(res["line_passes"], res["fused"]))
+    out["fused_over_line_passes"] = out["line_passes"]["ms"] / out["fused"]["ms"]
+    log("line kernel vs fused: %s" % out)
+    return out
+
+
 def section30_bench(P, hip, steps):
     """The slice-30 section (4,4)(4,3)(3,4)(4,4) (standard form (3,4)(4,4)(4,4)(4,3), shape
     318 x 160 x 160 x 318): a box of 10 x 20 x 20 x 10 blocks (all blocks along the two
@@ -503,6 +536,8 @@ def main():
                     help="skip the slice-30 section box measurement")
     ap.add_argument("--no-slices", dest="slices", action="store_false",
                     help="skip the whole-slice (35, 34, 33) measurement on real data")
+    ap.add_argument("--no-line-kernel", dest="line_kernel", action="store_false",
+                    help="skip the line-pass (schedule_compute_line form) comparison box")
     ap.add_argument("--no-slice32", dest="slice32", action="store_false",
                     help="skip slice 32 (computed in groups with regenerated children, ~13 s)")
     args = ap.parse_args()
@@ -603,6 +638,8 @@ def main():
         del outs, kids
     if not args.no_end and args.section30:
         result["section30"] = section30_bench(P, hip, args.steps)
+    if not args.no_end and args.line_kernel:
+        result["line_kernel"] = line_kernel_bench(P, hip, args.steps)
     if not args.no_end and args.slices:
         result["slices"] = slices_bench(P, hip, slice32=args.slice32)
     if not args.no_mid:
