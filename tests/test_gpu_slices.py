@@ -174,3 +174,44 @@ def test_slice_32_matches_production_counts():
     release(34)
     assert int(counts[:, 3].sum()) == 0
     assert np.array_equal(P.counts_rows(done, counts, 32), golden_rows(32))
+
+
+def test_plan_mixing_slice_34_and_35_outputs():
+    """One slab plan with a 35-stone output (line pass, no children) and 34-stone outputs
+    (fused kernel) equals the two computed separately."""
+    s35 = P.descendent_sections(EMPTY, 35)
+    s34 = P.descendent_sections(EMPTY, 34)
+    small34 = sorted(s34, key=lambda s: int(np.prod(P.section_shape(s))))[:4]
+    kids = sorted({c for s in small34 for c in S.child_sections(s)})
+    st35 = P.SliceStore(kids)
+    st35.compute()
+    extra35 = [s for s in s35 if s not in set(kids)][:1]
+    a34, b34 = P.SliceStore(small34), P.SliceStore(small34)
+    a35, b35 = P.SliceStore(extra35), P.SliceStore(extra35)
+    children = [st35.slab(k) for k in range(len(kids))]
+    # separately
+    a35.compute()
+    a34.compute(st35)
+    # together: one plan over both stores' slabs
+    plan = P.Plan.slab([b35.slab(0)] + [b34.slab(k) for k in range(len(small34))], children, 0)
+    plan.run()
+    hip.synchronize()
+    plan.close()
+    for x, y, secs in ((a34, b34, small34), (a35, b35, extra35)):
+        assert np.array_equal(x.counts(), y.counts())
+        for k, sec in enumerate(secs):
+            shape = P.section_shape(sec)
+            nb = [(v + 7) // 8 for v in shape]
+            n = int(np.prod(nb)) * 4096 * 8
+            got = np.zeros(n, dtype=np.uint64)
+            want = np.zeros(n, dtype=np.uint64)
+            for arr, store in ((got, y), (want, x)):
+                hip.check(hip.hip().hipMemcpy(hip.C.c_void_p(arr.ctypes.data), hip.C.c_void_p(store.slab(k).data),
+                                              arr.nbytes, hip.D2H), "hipMemcpy D2H")
+            # compare the valid prefix of every block slot
+            for b in range(int(np.prod(nb))):
+                blk = np.unravel_index(b, nb)
+                m = 8 * int(np.prod([min(8, shape[a] - 8 * blk[a]) for a in range(4)]))
+                assert np.array_equal(got[b * 4096 * 8:b * 4096 * 8 + m], want[b * 4096 * 8:b * 4096 * 8 + m]), (sec, b)
+    for s in (st35, a34, b34, a35, b35):
+        s.close()
