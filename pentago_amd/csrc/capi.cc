@@ -225,11 +225,24 @@ pe_status pe_end_plan_create(int n, const pe_line_job* jobs, pe_end_plan** out) 
       h.in_off.assign(J.input_block_offsets, J.input_block_offsets + h.info.input_blocks);
     if (J.output_block_offsets) h.out_off.assign(J.output_block_offsets, J.output_block_offsets + h.info.length);
   }
+  // jobs covering whole lines run on the fused kernel (one dimension open per block);
+  // partial index ranges, slice 35 and irregular child offsets on the line kernel
+  std::vector<LineHost> fused, lined;
+  for (auto& h : hosts) (fused_line_eligible(h) ? fused : lined).push_back(std::move(h));
   auto* plan = new pe_end_plan();
-  plan->passes.resize(1);
-  const pe_status s = plan_create(hosts, &plan->passes[0]);
-  if (s != PE_OK) { delete plan; return s; }
-  plan->nodes = plan->passes[0].output_nodes;
+  std::string err;
+  pe_status s = PE_OK;
+  if (!fused.empty()) {
+    s = fused_line_plan_create(fused, &plan->fused, &err);
+    if (s != PE_OK) { delete plan; return fail(s, err); }
+    plan->nodes += plan->fused.output_nodes;
+  }
+  if (!lined.empty()) {
+    plan->passes.resize(1);
+    s = plan_create(lined, &plan->passes[0]);
+    if (s != PE_OK) { fused_plan_destroy(&plan->fused); delete plan; return s; }
+    plan->nodes += plan->passes[0].output_nodes;
+  }
   *out = plan;
   return PE_OK;
 }

@@ -138,6 +138,7 @@ pe_status make_line(const pe_line& line, LineHost* out, std::string* err) {
   std::memset(&out->dev, 0, sizeof out->dev);
   std::memset(&out->info, 0, sizeof out->info);
   out->refl_syms.clear();
+  out->section = line.section;
   LineDev& D = out->dev;
   pe_line_info& I = out->info;
   Section s;

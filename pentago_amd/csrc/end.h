@@ -64,6 +64,7 @@ struct LineDev {
 struct LineHost {
   LineDev dev;
   pe_line_info info;
+  pe_section section;                  // the line's (standard) section
   std::vector<uint8_t> refl_syms;      // all_reflection_symmetries (compute.cc:105-122)
   std::vector<int64_t> in_off, out_off; // optional block offset tables
 };
@@ -96,8 +97,9 @@ struct SecDev {
 struct BlockDev {
   int32_t sec;
   int32_t b[4];
-  int32_t line[4];          // -1 if the dimension is closed
-  uint64_t* out;            // the block's 4096-node slot in its output slab
+  int32_t line[4];          // -1 if the dimension is closed (or not computed)
+  int32_t accumulate;       // OR into the block's existing output instead of storing
+  uint64_t* out;            // the block's output: C-order over its shape
 };
 
 struct FusedPlan {
@@ -106,6 +108,13 @@ struct FusedPlan {
   size_t off_lines = 0, off_secs = 0, off_fields = 0, off_moves = 0, off_refl = 0;
   int64_t output_nodes = 0;
 };
+
+// Fused plan over block lines (one dimension each, the reference's line buffers): every
+// job must cover its whole line (all indices along the dimension), have child blocks at
+// evenly spaced offsets and at most 34 stones; jobs that do not qualify stay on the line
+// kernel (fused_line_eligible).
+bool fused_line_eligible(const LineHost& h);
+pe_status fused_line_plan_create(const std::vector<LineHost>& jobs, FusedPlan* plan, std::string* err);
 
 // Field tables of one section (see SecDev): for quadrant q and rotation-minimal index i,
 // 60 words: for each super word k, tables [F0 (mover's side), F1 (other side), Fm (OR over

@@ -228,6 +228,10 @@ void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __r
     PE_FENCE();
   }
   uint64_t* o = B.out + 8 * p;
+  if (B.accumulate) {   // OR-merge with other dimensions' lines (block_store.cc:173-229)
+    o0 = s4_or(o0, ld(o));
+    o1 = s4_or(o1, ld(o + 4));
+  }
   st(o, o0);
   st(o + 4, o1);
 }
@@ -309,68 +313,12 @@ static pe_status slab_lines(const pe_slab& out, int nch, const pe_slab* ch, std:
   return PE_OK;
 }
 
-pe_status fused_plan_create(int nout, const pe_slab* outs, int nch, const pe_slab* ch, FusedPlan* plan,
-                            std::string* err) {
-  DevTables T;
-  if (device_tables(&T, err) != PE_OK) return PE_ERR_DEVICE;
-  const auto& Tb = tables();
-  std::vector<LineHost> lines;
-  std::vector<SecDev> secs;
-  std::vector<uint64_t> fields;
-  std::vector<uint16_t> movetab;
-  std::vector<BlockDev> blocks;
-  int64_t nodes = 0;
-  for (int k = 0; k < nout; k++) {
-    const pe_slab& out = outs[k];
-    if (!out.data) { *err = "output slab without memory"; return PE_ERR_INVALID; }
-    std::vector<LineHost> sl;
-    std::map<std::tuple<int, int, int, int>, int> index;
-    pe_status st = slab_lines(out, nch, ch, &sl, &index, err);
-    if (st != PE_OK) return st;
-    Section s;
-    std::memcpy(s.c, out.section.counts, 8);
-    if (s.sum() > 34) { *err = "the fused kernel needs sections with at most 34 stones"; return PE_ERR_INVALID; }
-    SecDev sec;
-    std::memset(&sec, 0, sizeof sec);
-    sec.turn = s.sum() & 1;
-    std::vector<uint64_t> f;
-    section_field_tables(s, &f, sec.field_off);
-    std::vector<uint16_t> mt;
-    st = section_move_tables(s, &mt, sec.move_off, sec.nmoves, err);
-    if (st != PE_OK) return st;
-    for (int a = 0; a < 4; a++) {
-      sec.field_off[a] += int32_t(fields.size() / 4);
-      if (sec.move_off[a] >= 0) sec.move_off[a] += int32_t(movetab.size());
-      const int b = rmin_bucket(s.c[a][0], s.c[a][1]);
-      sec.rmin_off[a] = Tb.rmin_offsets[b];
-      sec.rmin_size[a] = Tb.rmin_offsets[b + 1] - Tb.rmin_offsets[b];
-    }
-    fields.insert(fields.end(), f.begin(), f.end());
-    movetab.insert(movetab.end(), mt.begin(), mt.end());
-    const int line0 = int(lines.size());
-    for (auto& h : sl) lines.push_back(std::move(h));
-    int b[4];
-    for (b[0] = out.lo[0]; b[0] < out.hi[0]; b[0]++)
-      for (b[1] = out.lo[1]; b[1] < out.hi[1]; b[1]++)
-        for (b[2] = out.lo[2]; b[2] < out.hi[2]; b[2]++)
-          for (b[3] = out.lo[3]; b[3] < out.hi[3]; b[3]++) {
-            BlockDev B;
-            B.sec = int32_t(secs.size());
-            int n = 1;
-            for (int a = 0; a < 4; a++) { B.b[a] = b[a]; n *= std::min(8, sec.rmin_size[a] - 8 * b[a]); }
-            for (int d = 0; d < 4; d++) {
-              B.line[d] = -1;
-              if (s.c[d][0] + s.c[d][1] >= 9) continue;
-              int o[3], kk = 0;
-              for (int a = 0; a < 4; a++) if (a != d) o[kk++] = b[a];
-              B.line[d] = line0 + index.at(std::make_tuple(d, o[0], o[1], o[2]));
-            }
-            B.out = static_cast<uint64_t*>(out.data) + 8 * 4096 * slab_index(out, b);
-            blocks.push_back(B);
-            nodes += n;
-          }
-    secs.push_back(sec);
-  }
+// Pack blocks, lines (with their child offsets and reflection symmetries), sections, field
+// and move tables into one device allocation.
+static pe_status pack_fused_plan(const std::vector<BlockDev>& blocks, const std::vector<LineHost>& lines,
+                                 const std::vector<SecDev>& secs, const std::vector<uint64_t>& fields,
+                                 const std::vector<uint16_t>& movetab, int64_t nodes, FusedPlan* plan,
+                                 std::string* err) {
   if (blocks.size() > size_t(0x7fffffff) / kParts) { *err = "fused plan too large"; return PE_ERR_CAPACITY; }
   // blob: blocks | lines | sections | fields | moves | offsets | symmetries
   std::vector<LineDev> devs(lines.size());
@@ -428,6 +376,157 @@ pe_status fused_plan_create(int nout, const pe_slab* outs, int nch, const pe_sla
   plan->off_refl = o_refl;
   plan->output_nodes = nodes;
   return PE_OK;
+}
+
+pe_status fused_plan_create(int nout, const pe_slab* outs, int nch, const pe_slab* ch, FusedPlan* plan,
+                            std::string* err) {
+  DevTables T;
+  if (device_tables(&T, err) != PE_OK) return PE_ERR_DEVICE;
+  const auto& Tb = tables();
+  std::vector<LineHost> lines;
+  std::vector<SecDev> secs;
+  std::vector<uint64_t> fields;
+  std::vector<uint16_t> movetab;
+  std::vector<BlockDev> blocks;
+  int64_t nodes = 0;
+  for (int k = 0; k < nout; k++) {
+    const pe_slab& out = outs[k];
+    if (!out.data) { *err = "output slab without memory"; return PE_ERR_INVALID; }
+    std::vector<LineHost> sl;
+    std::map<std::tuple<int, int, int, int>, int> index;
+    pe_status st = slab_lines(out, nch, ch, &sl, &index, err);
+    if (st != PE_OK) return st;
+    Section s;
+    std::memcpy(s.c, out.section.counts, 8);
+    if (s.sum() > 34) { *err = "the fused kernel needs sections with at most 34 stones"; return PE_ERR_INVALID; }
+    SecDev sec;
+    std::memset(&sec, 0, sizeof sec);
+    sec.turn = s.sum() & 1;
+    std::vector<uint64_t> f;
+    section_field_tables(s, &f, sec.field_off);
+    std::vector<uint16_t> mt;
+    st = section_move_tables(s, &mt, sec.move_off, sec.nmoves, err);
+    if (st != PE_OK) return st;
+    for (int a = 0; a < 4; a++) {
+      sec.field_off[a] += int32_t(fields.size() / 4);
+      if (sec.move_off[a] >= 0) sec.move_off[a] += int32_t(movetab.size());
+      const int b = rmin_bucket(s.c[a][0], s.c[a][1]);
+      sec.rmin_off[a] = Tb.rmin_offsets[b];
+      sec.rmin_size[a] = Tb.rmin_offsets[b + 1] - Tb.rmin_offsets[b];
+    }
+    fields.insert(fields.end(), f.begin(), f.end());
+    movetab.insert(movetab.end(), mt.begin(), mt.end());
+    const int line0 = int(lines.size());
+    for (auto& h : sl) lines.push_back(std::move(h));
+    int b[4];
+    for (b[0] = out.lo[0]; b[0] < out.hi[0]; b[0]++)
+      for (b[1] = out.lo[1]; b[1] < out.hi[1]; b[1]++)
+        for (b[2] = out.lo[2]; b[2] < out.hi[2]; b[2]++)
+          for (b[3] = out.lo[3]; b[3] < out.hi[3]; b[3]++) {
+            BlockDev B;
+            B.sec = int32_t(secs.size());
+            B.accumulate = 0;
+            int n = 1;
+            for (int a = 0; a < 4; a++) { B.b[a] = b[a]; n *= std::min(8, sec.rmin_size[a] - 8 * b[a]); }
+            for (int d = 0; d < 4; d++) {
+              B.line[d] = -1;
+              if (s.c[d][0] + s.c[d][1] >= 9) continue;
+              int o[3], kk = 0;
+              for (int a = 0; a < 4; a++) if (a != d) o[kk++] = b[a];
+              B.line[d] = line0 + index.at(std::make_tuple(d, o[0], o[1], o[2]));
+            }
+            B.out = static_cast<uint64_t*>(out.data) + 8 * 4096 * slab_index(out, b);
+            blocks.push_back(B);
+            nodes += n;
+          }
+    secs.push_back(sec);
+  }
+  return pack_fused_plan(blocks, lines, secs, fields, movetab, nodes, plan, err);
+}
+
+bool fused_line_eligible(const LineHost& h) {
+  if (h.dev.slice35) return false;
+  if (h.dev.i_begin != 0 || h.dev.i_end != h.dev.rmin_size[h.dev.dim]) return false;
+  const std::vector<int64_t>& io = h.in_off;
+  for (size_t k = 2; k < io.size(); k++)
+    if (io[k] - io[k - 1] != io[1] - io[0]) return false;
+  return true;
+}
+
+// Each job's blocks become fused blocks with only the job's dimension open: the section's
+// field table with the move fields (Fm) of the other quadrants cleared, and only that
+// dimension's move table, so a block computes exactly the line's contribution
+// (compute.cc:219-371 for one dimension); accumulate jobs OR it into the output.
+pe_status fused_line_plan_create(const std::vector<LineHost>& jobs, FusedPlan* plan, std::string* err) {
+  DevTables T;
+  if (device_tables(&T, err) != PE_OK) return PE_ERR_DEVICE;
+  const auto& Tb = tables();
+  std::vector<LineHost> lines;
+  std::vector<SecDev> secs;
+  std::vector<uint64_t> fields;
+  std::vector<uint16_t> movetab;
+  std::vector<BlockDev> blocks;
+  std::map<std::pair<uint64_t, int>, int> sec_index;   // (section sig, dimension) -> SecDev
+  int64_t nodes = 0;
+  for (const LineHost& job : jobs) {
+    if (!fused_line_eligible(job)) { *err = "internal: line job not eligible for the fused kernel"; return PE_ERR_INVALID; }
+    const int d = job.dev.dim;
+    Section own;   // the line's own section
+    std::memcpy(own.c, job.section.counts, 8);
+    const auto key = std::make_pair(own.sig(), d);
+    auto it = sec_index.find(key);
+    if (it == sec_index.end()) {
+      SecDev sec;
+      std::memset(&sec, 0, sizeof sec);
+      sec.turn = own.sum() & 1;
+      std::vector<uint64_t> f;
+      section_field_tables(own, &f, sec.field_off);
+      // clear the move fields of every quadrant but d: only d's moves are this line's
+      for (int q = 0; q < 4; q++) {
+        if (q == d) continue;
+        const int n = rmin_count(own.c[q][0], own.c[q][1]);
+        for (int i = 0; i < n; i++)
+          for (int k = 0; k < 4; k++)
+            for (int fld = 0; fld < 5; fld++) f[size_t(4 * sec.field_off[q] + 60 * i + 15 * k + 10 + fld)] = 0;
+      }
+      std::vector<uint16_t> mt;
+      pe_status st = section_move_tables(own, &mt, sec.move_off, sec.nmoves, err);
+      if (st != PE_OK) return st;
+      for (int a = 0; a < 4; a++) {
+        sec.field_off[a] += int32_t(fields.size() / 4);
+        if (a != d) { sec.move_off[a] = -1; sec.nmoves[a] = 0; }
+        else sec.move_off[a] += int32_t(movetab.size());
+        const int b = rmin_bucket(own.c[a][0], own.c[a][1]);
+        sec.rmin_off[a] = Tb.rmin_offsets[b];
+        sec.rmin_size[a] = Tb.rmin_offsets[b + 1] - Tb.rmin_offsets[b];
+      }
+      fields.insert(fields.end(), f.begin(), f.end());
+      movetab.insert(movetab.end(), mt.begin(), mt.end());
+      it = sec_index.emplace(key, int(secs.size())).first;
+      secs.push_back(sec);
+    }
+    const SecDev& sec = secs[size_t(it->second)];
+    LineHost h = job;
+    if (h.in_off.empty())
+      for (int k = 0; k < h.info.input_blocks; k++) h.in_off.push_back(int64_t(k) * (h.dev.block_stride + 1));
+    const int line_index = int(lines.size());
+    for (int k = 0; k < h.dev.length; k++) {
+      BlockDev B;
+      B.sec = it->second;
+      int n = 1;
+      for (int a = 0; a < 4; a++) {
+        B.b[a] = a == d ? k : h.dev.first_block[a];
+        n *= std::min(8, sec.rmin_size[a] - 8 * B.b[a]);
+        B.line[a] = a == d ? line_index : -1;
+      }
+      B.accumulate = h.dev.accumulate;
+      B.out = h.dev.output + 8 * (h.out_off.empty() ? int64_t(k) * h.dev.block_stride : h.out_off[size_t(k)]);
+      blocks.push_back(B);
+      nodes += n;
+    }
+    lines.push_back(std::move(h));
+  }
+  return pack_fused_plan(blocks, lines, secs, fields, movetab, nodes, plan, err);
 }
 
 pe_status fused_plan_run(const FusedPlan& plan, hipStream_t stream) {
