@@ -549,12 +549,16 @@ pe_status pe_mid_autotune(int n, const pe_high_board* boards, int64_t workspace_
       if (st != PE_OK) break;
       MidPlan plan;
       st = mid_plan_create(infos, cfg, &plan);
-      float ms = 0;
+      float ms = -1;
       if (st == PE_OK) st = mid_plan_run(plan, s);   // warm-up
-      if (st == PE_OK && hipEventRecord(e0, s) == hipSuccess) st = mid_plan_run(plan, s);
-      if (st == PE_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                          hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-        st = fail(PE_ERR_DEVICE, "autotune timing failed");
+      for (int rep = 0; rep < 2 && st == PE_OK; rep++) {   // the faster of two timed runs
+        float t = 0;
+        if (hipEventRecord(e0, s) == hipSuccess) st = mid_plan_run(plan, s);
+        if (st == PE_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+          st = fail(PE_ERR_DEVICE, "autotune timing failed");
+        if (st == PE_OK && (ms < 0 || t < ms)) ms = t;
+      }
       (void)hipStreamSynchronize(s);
       mid_plan_destroy(&plan);
       if (st != PE_OK) break;

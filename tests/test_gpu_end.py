@@ -196,3 +196,57 @@ def test_production_samples_on_gpu(slice_):
         expect_b = sum(wins[i] << (64 * i) for i in range(4))
         expect_w = sum(wins[4 + i] << (64 * i) for i in range(4))
         assert (black, white) == (expect_b, expect_w), board
+
+
+@pytest.mark.parametrize("case", [1, 5, 9])
+def test_line_jobs_offsets_accumulate_and_ranges(case):
+    """Line-job plans with scattered child and output blocks (reversed, gapped), OR-
+    accumulating into a prefilled output: the whole-line job (fused kernel) and a job
+    restricted to indices 1.. (line kernel) against the oracle's line."""
+    counts, dim, where = CASES[case]
+    pl, ol = make_case(counts, dim, where)
+    info = P.line_describe(pl)
+    inp = line_input(info, 70 + case)
+    ref = np.zeros(8 * info.output_nodes, dtype=np.uint64)
+    O.compute_lines_threaded([ol], [inp], [ref], 16)
+    bs, nin, nout = info.block_stride, info.input_blocks, info.length
+    # child block k at node (nin - 1 - k) * (bs + 9): evenly spaced, descending, with gaps
+    in_off = [(nin - 1 - k) * (bs + 9) for k in range(nin)]
+    dev_in = np.zeros(8 * (nin * (bs + 9)), dtype=np.uint64)
+    for k in range(nin):
+        src = inp[8 * k * (bs + 1):8 * (k * (bs + 1) + bs + 1)]
+        dev_in[8 * in_off[k]:8 * in_off[k] + src.size] = src
+    # output block k at node (nout - 1 - k) * (bs + 3)
+    out_off = [(nout - 1 - k) * (bs + 3) for k in range(nout)]
+    prefill = seeded_nodes(nout * (bs + 3), 90 + case)
+    din = hip.Buffer(dev_in.nbytes)
+    din.upload(dev_in)
+    # the index along the line of every output node (reference layout, per block)
+    oshape = list(info.output_shape)
+    along = np.zeros(info.output_nodes, dtype=np.int64)
+    for k in range(nout):
+        bshape = list(oshape)
+        bshape[dim] = min(8, oshape[dim] - 8 * k)
+        idx = np.indices(bshape)[dim].reshape(-1) + 8 * k
+        along[k * bs:k * bs + idx.size] = idx
+    for i_range in (None, (1, oshape[dim])):
+        dout = hip.Buffer(prefill.nbytes)
+        dout.upload(prefill)
+        job = dict(line=pl, input=din.ptr, output=dout.ptr, in_offsets=in_off, out_offsets=out_off, accumulate=1)
+        if i_range:
+            job["i_range"] = i_range
+        plan = P.Plan([job])
+        plan.run()
+        hip.synchronize()
+        plan.close()
+        got = dout.download(np.zeros_like(prefill))
+        want = prefill.copy()
+        for k in range(nout):
+            n = min(bs, info.output_nodes - k * bs)
+            sel = np.arange(k * bs, k * bs + n)
+            if i_range:
+                sel = sel[along[sel] >= i_range[0]]
+            for j in sel:
+                o = out_off[k] + (j - k * bs)
+                want[8 * o:8 * o + 8] |= ref[8 * j:8 * j + 8]
+        assert np.array_equal(got, want), (case, i_range)
