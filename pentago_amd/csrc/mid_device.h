@@ -147,28 +147,57 @@ __device__ inline void mid_position(const MidBoardDev& B, int n, int64_t pos64, 
   const int s1p = int(pos - s0 * nsets1p);
   const Set0Info& I0 = B.set0[I.sets0_off[n] + int(s0)];
   const uint32_t filled1p = B.set1p_bits[I.sets1p_off[n] + s1p];
-  uint16_t s1 = 0, s0p = 0;
+  // members in increasing order (member i is q), four at a time so that their table reads
+  // are in flight together
+  uint32_t s1w = 0, s0pw = 0;
   {
     uint32_t rest = filled1p;
-    for (int i = 0; rest; i++) {   // members in increasing order: member i is q
-      const int q = __builtin_ctz(rest);
-      rest &= rest - 1;
-      s1 = uint16_t(s1 + C[I0.empty1[q] * 32 + i + 1]);
-      s0p = uint16_t(s0p + I0.offset0[i * (spots - k0) + q]);
+    for (int i0 = 0; rest; i0 += 4) {
+      int qs[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        qs[u] = rest ? __builtin_ctz(rest) : -1;
+        rest &= rest - 1;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++)
+        if (qs[u] >= 0) {
+          s1w += uint32_t(C[I0.empty1[qs[u]] * 32 + i0 + u + 1]);
+          s0pw += uint32_t(I0.offset0[(i0 + u) * (spots - k0) + qs[u]]);
+        }
     }
   }
+  const uint16_t s1 = uint16_t(s1w), s0p = uint16_t(s0pw);
   uint64_t win0 = 0, win1 = 0, nl0 = 0, nl1 = 0;
   if (n == spots) nl0 = nl1 = ~uint64_t(0);
   const MidGrab gin = I.spaces[n + 1];
   const uint16_t* cs = B.cs1ps + I.cs1ps_off[n] + s1p * (spots - k0);
+  // children: the unfilled cells in increasing order, four gathers issued before any is used
   uint32_t unmoved = ~filled1p;
-  for (int m = 0; m < spots - n; m++) {
-    const int i = __builtin_ctz(unmoved);
-    unmoved &= unmoved - 1;
-    const uint64_t* c = work + 4 * (int64_t(gin.lo) + int64_t(I0.child_s0s[i]) * gin.ny + cs[i]);
-    const ulonglong2 a = reinterpret_cast<const ulonglong2*>(c)[0];
-    const ulonglong2 b = reinterpret_cast<const ulonglong2*>(c)[1];
-    win0 |= a.x; win1 |= a.y; nl0 |= b.x; nl1 |= b.y;
+  const int nm = spots - n;
+  for (int m0 = 0; m0 < nm; m0 += 4) {
+    const uint64_t* cp[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int i = __builtin_ctz(unmoved);
+      unmoved &= unmoved - 1;
+      cp[u] = m0 + u < nm ? work + 4 * (int64_t(gin.lo) + int64_t(I0.child_s0s[i]) * gin.ny + cs[i]) : nullptr;
+    }
+    ulonglong2 a[4], b[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (cp[u]) {
+        a[u] = reinterpret_cast<const ulonglong2*>(cp[u])[0];
+        b[u] = reinterpret_cast<const ulonglong2*>(cp[u])[1];
+      } else {
+        a[u] = make_ulonglong2(0, 0);
+        b[u] = make_ulonglong2(0, 0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      win0 |= a[u].x; win1 |= a[u].y; nl0 |= b[u].x; nl1 |= b[u].y;
+    }
   }
   const Wins1 w1 = B.wins1[I.wins1_off[n] + s1];
   const uint64_t ip0 = ~(I0.wins0[0] | w1.after[0]), ip1 = ~(I0.wins0[1] | w1.after[1]);
