@@ -17,7 +17,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     text = open(os.path.join(ROOT, "include", "pentago_amd.h")).read()
-    return sorted(set(re.findall(r"^(?:pe_status|int64_t|const char\*)\s+(pe_[a-z_0-9]+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^(?:pe_status|int64_t|int|const char\*)\s+(pe_[a-z_0-9]+)\s*\(", text, re.M)))
 
 
 def test_exports_every_declared_symbol():
