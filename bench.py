@@ -264,7 +264,8 @@ def slices_bench(P, hip, budget=None, slice32=False):
     """Real data: slices 35, 34 and 33 (every section the empty board reaches) computed on
     the device from nothing, slice 33 in groups of sections beside slice 34; kernel time
     from HIP events on the launch stream, plan build time apart, the count kernel timed over
-    each slice, and every section's counts compared with the production run's."""
+    each slice (one launch per group of sections, the descriptor upload inside the timed
+    region), and every section's counts compared with the production run's."""
     empty = ((0, 0),) * 4
     golden = os.path.join(ROOT, "tests", "golden")
     stream = hip.Stream()
@@ -287,8 +288,9 @@ def slices_bench(P, hip, budget=None, slice32=False):
         kernel = timed(lambda: plan.run(stream.ptr))
         plan.close()
         d_counts = hip.Buffer(32 * sum(s.blocks for s in slabs))
+        ws = hip.Buffer(P.count_workspace_bytes(len(slabs)))
         offs = np.cumsum([0] + [s.blocks for s in slabs])
-        count = timed(lambda: [P.count_blocks(s, d_counts.ptr + 32 * int(o), stream.ptr) for s, o in zip(slabs, offs)])
+        count = timed(lambda: P.count_blocks_many(slabs, d_counts.ptr, ws.ptr, stream.ptr))
         per_block = d_counts.download(np.zeros((int(offs[-1]), 4), dtype=np.uint64))
         counts = np.array([per_block[offs[k]:offs[k + 1]].sum(axis=0) for k in range(len(slabs))], dtype=np.uint64)
         return st, counts, build, kernel, count

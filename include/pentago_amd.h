@@ -164,6 +164,13 @@ pe_status pe_fill_seeded(void* d_nodes, int64_t nodes, uint64_t seed, int64_t fi
 pe_status pe_end_count_blocks(const pe_slab* slab, uint64_t* d_counts, void* stream);
 /* The same summed per slab, synchronous: counts[k] = totals over slab k's blocks. */
 pe_status pe_end_count_slabs(int n, const pe_slab* slabs, uint64_t (*counts)[4]);
+/* pe_end_count_blocks over n slabs in one kernel launch: slab k's per-block counts start at
+ * d_counts + 4 * (blocks of slabs 0..k-1).  d_workspace is device memory of
+ * pe_end_count_workspace_bytes(n) bytes for the launch descriptors; it must stay unused by
+ * anything else until the kernel has run.  Returns once the descriptors are uploaded (which
+ * first waits for the work already on `stream`); the kernel runs asynchronously. */
+pe_status pe_end_count_blocks_many(int n, const pe_slab* slabs, uint64_t* d_counts, void* d_workspace, void* stream);
+int64_t pe_end_count_workspace_bytes(int n);
 
 /* Sparse samples: the node of each board (a rotation-standardized board of the slab's
  * section inside the slab's box; quadrant q in bits 16q..16q+15, radix 3) copied to

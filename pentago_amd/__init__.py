@@ -153,6 +153,8 @@ def lib():
             "pe_fill_seeded": (C.c_int, [V, C.c_int64, C.c_uint64, C.c_int64, V]),
             "pe_end_count_blocks": (C.c_int, [P(Slab), V, V]),
             "pe_end_count_slabs": (C.c_int, [C.c_int, P(Slab), V]),
+            "pe_end_count_blocks_many": (C.c_int, [C.c_int, P(Slab), V, V, V]),
+            "pe_end_count_workspace_bytes": (C.c_int64, [C.c_int]),
             "pe_end_slab_samples": (C.c_int, [P(Slab), C.c_int, P(C.c_uint64), V]),
             "pe_supertensor_decode": (C.c_int, [C.c_int, C.c_int, V, C.c_int64, V, V]),
             "pe_supertensor_encode": (C.c_int, [C.c_int, C.c_int, V, C.c_int64, V, V]),
@@ -347,6 +349,19 @@ def count_slabs(slabs):
 def count_blocks(slab, d_counts, stream=None):
     """Per-block counts of one slab into device memory (4 uint64 per block), asynchronous."""
     _check(lib().pe_end_count_blocks(C.byref(slab), d_counts, stream), "pe_end_count_blocks")
+
+
+def count_blocks_many(slabs, d_counts, d_workspace, stream=None):
+    """Per-block counts of many slabs in one launch into device memory (4 uint64 per block,
+    slab after slab); d_workspace holds count_workspace_bytes(len(slabs)) bytes."""
+    slabs = list(slabs)
+    arr = (Slab * max(len(slabs), 1))(*slabs)
+    _check(lib().pe_end_count_blocks_many(len(slabs), arr, d_counts, d_workspace, stream),
+           "pe_end_count_blocks_many")
+
+
+def count_workspace_bytes(n):
+    return int(lib().pe_end_count_workspace_bytes(int(n)))
 
 
 def slab_samples(slab, boards):

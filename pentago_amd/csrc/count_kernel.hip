@@ -33,6 +33,7 @@ struct CountArgs {
   uint64_t* counts;          // 4 u64 per block of the box
   int32_t lo[4], ext[4];     // box origin and extent, in blocks
   int32_t rmin_off[4], rmin_size[4];
+  int64_t first;             // index of the slab's first block among all blocks of the launch
 };
 
 __device__ inline uint32_t wave_sum(uint32_t v) {
@@ -40,8 +41,8 @@ __device__ inline uint32_t wave_sum(uint32_t v) {
   return v;
 }
 
-__global__ __launch_bounds__(kCountThreads) void count_blocks_kernel(CountArgs A, const uint8_t* __restrict__ stab) {
-  const int64_t blk = blockIdx.x;
+// Block blk of slab A, by one workgroup.
+__device__ inline void count_block(const CountArgs& A, int64_t blk, const uint8_t* __restrict__ stab) {
   int b[4], n[4];
   int64_t r = blk;
   for (int a = 3; a >= 0; a--) {
@@ -91,44 +92,99 @@ __global__ __launch_bounds__(kCountThreads) void count_blocks_kernel(CountArgs A
   }
 }
 
-}  // namespace
-
-// Per-block counts of one slab into d_counts (4 u64 per block, C-order over the box).
-pe_status count_slab_blocks(const pe_slab& slab, uint64_t* d_counts, hipStream_t stream, std::string* err) {
-  DevTables T;
-  if (device_tables(&T, err) != PE_OK) return PE_ERR_DEVICE;
-  Section s;
-  std::memcpy(s.c, slab.section.counts, 8);
-  int shape[4];
-  if (!s.valid_counts() || !s.shape(shape)) { *err = "invalid slab section"; return PE_ERR_INVALID; }
-  if (!slab.data || !d_counts) { *err = "null slab data or counts"; return PE_ERR_INVALID; }
-  CountArgs A;
-  A.data = static_cast<const uint64_t*>(slab.data);
-  A.counts = d_counts;
-  int64_t nblocks = 1;
-  const auto& Tb = tables();
-  for (int a = 0; a < 4; a++) {
-    if (slab.lo[a] < 0 || slab.hi[a] <= slab.lo[a] || 8 * (slab.hi[a] - 1) >= shape[a]) {
-      *err = "slab box outside its section";
-      return PE_ERR_INVALID;
-    }
-    A.lo[a] = slab.lo[a];
-    A.ext[a] = slab.hi[a] - slab.lo[a];
-    nblocks *= A.ext[a];
-    const int bucket = rmin_bucket(s.c[a][0], s.c[a][1]);
-    A.rmin_off[a] = Tb.rmin_offsets[bucket];
-    A.rmin_size[a] = shape[a];
-  }
-  if (nblocks > 0x7fffffff) { *err = "slab too large"; return PE_ERR_CAPACITY; }
-  hipLaunchKernelGGL(count_blocks_kernel, dim3(unsigned(nblocks)), dim3(kCountThreads), 0, stream, A, T.rmin_stab);
-  if (hipGetLastError() != hipSuccess) { *err = "count_blocks_kernel launch failed"; return PE_ERR_DEVICE; }
-  return PE_OK;
+// One slab, its descriptor passed by value: workgroup k counts block k.
+__global__ __launch_bounds__(kCountThreads) void count_blocks_kernel(CountArgs A, const uint8_t* __restrict__ stab) {
+  count_block(A, blockIdx.x, stab);
 }
+
+// Many slabs in one launch: slab s owns workgroups [first_s, first_{s+1}) (first ascending,
+// first_0 = 0).  Each wave finds its slab by a 64-way search: the lanes test 64 evenly spaced
+// descriptors of the remaining range at once, so 4096 slabs take two dependent loads.
+__global__ __launch_bounds__(kCountThreads) void count_slabs_kernel(const CountArgs* __restrict__ slabs, int nslabs,
+                                                                     const uint8_t* __restrict__ stab) {
+  const int64_t wg = blockIdx.x;
+  const int lane = threadIdx.x & 63;
+  int lo = 0, len = nslabs;   // the slab is in [lo, lo + len) and first_lo <= wg
+  while (len > 1) {
+    const int step = (len + 63) / 64;
+    const int idx = lo + lane * step;
+    const bool le = idx < lo + len && slabs[idx].first <= wg;
+    const int c = __popcll(__ballot(le));   // a prefix of the lanes, lane 0 among them
+    const int end = lo + len;
+    lo += (c - 1) * step;
+    len = min(step, end - lo);
+  }
+  const CountArgs A = slabs[lo];
+  count_block(A, wg - A.first, stab);
+}
+
+}  // namespace
 
 int64_t slab_blocks(const pe_slab& slab) {
   int64_t n = 1;
   for (int a = 0; a < 4; a++) n *= std::max(0, slab.hi[a] - slab.lo[a]);
   return n;
 }
+
+// Launch descriptor of one slab (counts land at d_counts, 4 u64 per block).
+static pe_status count_args(const pe_slab& slab, uint64_t* d_counts, int64_t first, CountArgs* A, std::string* err) {
+  Section s;
+  std::memcpy(s.c, slab.section.counts, 8);
+  int shape[4];
+  if (!s.valid_counts() || !s.shape(shape)) { *err = "invalid slab section"; return PE_ERR_INVALID; }
+  if (!slab.data || !d_counts) { *err = "null slab data or counts"; return PE_ERR_INVALID; }
+  A->data = static_cast<const uint64_t*>(slab.data);
+  A->counts = d_counts;
+  A->first = first;
+  const auto& Tb = tables();
+  for (int a = 0; a < 4; a++) {
+    if (slab.lo[a] < 0 || slab.hi[a] <= slab.lo[a] || 8 * (slab.hi[a] - 1) >= shape[a]) {
+      *err = "slab box outside its section";
+      return PE_ERR_INVALID;
+    }
+    A->lo[a] = slab.lo[a];
+    A->ext[a] = slab.hi[a] - slab.lo[a];
+    const int bucket = rmin_bucket(s.c[a][0], s.c[a][1]);
+    A->rmin_off[a] = Tb.rmin_offsets[bucket];
+    A->rmin_size[a] = shape[a];
+  }
+  return PE_OK;
+}
+
+// Per-block counts of n slabs in one launch: slab k's blocks land at d_counts[k] (4 u64 per
+// block, C-order over its box).  With desc null (one slab only) the descriptor is passed by
+// value and the call is asynchronous; otherwise the descriptors go up into `desc`, device
+// memory of count_desc_bytes(n) bytes, by a copy that waits for the stream first.
+pe_status count_blocks_launch(int n, const pe_slab* slabs, uint64_t* const* d_counts, void* desc, hipStream_t stream,
+                              std::string* err) {
+  DevTables T;
+  if (device_tables(&T, err) != PE_OK) return PE_ERR_DEVICE;
+  std::vector<CountArgs> args(size_t(std::max(n, 0)));
+  int64_t total = 0;
+  for (int k = 0; k < n; k++) {
+    const pe_status st = count_args(slabs[k], d_counts[k], total, &args[size_t(k)], err);
+    if (st != PE_OK) return st;
+    total += slab_blocks(slabs[k]);
+  }
+  if (!total) return PE_OK;
+  if (total > 0x7fffffff) { *err = "too many blocks in one count launch"; return PE_ERR_CAPACITY; }
+  if (n == 1 && !desc) {
+    hipLaunchKernelGGL(count_blocks_kernel, dim3(unsigned(total)), dim3(kCountThreads), 0, stream, args[0], T.rmin_stab);
+  } else {
+    if (!desc) { *err = "no descriptor memory for a multi-slab count"; return PE_ERR_INVALID; }
+    // Synchronous: the descriptors live on this function's stack frame.
+    if (hipStreamSynchronize(stream) != hipSuccess ||
+        hipMemcpy(desc, args.data(), sizeof(CountArgs) * args.size(), hipMemcpyHostToDevice) != hipSuccess) {
+      *err = "count descriptor upload failed";
+      return PE_ERR_DEVICE;
+    }
+    hipLaunchKernelGGL(count_slabs_kernel, dim3(unsigned(total)), dim3(kCountThreads), 0, stream,
+                       static_cast<const CountArgs*>(desc), n, T.rmin_stab);
+  }
+  if (hipGetLastError() != hipSuccess) { *err = "count kernel launch failed"; return PE_ERR_DEVICE; }
+  return PE_OK;
+}
+
+size_t count_desc_bytes(int n) { return sizeof(CountArgs) * size_t(std::max(n, 1)); }
 
 }  // namespace pe

@@ -18,7 +18,9 @@
 #include "tables.h"
 
 namespace pe {
-pe_status count_slab_blocks(const pe_slab& slab, uint64_t* d_counts, hipStream_t stream, std::string* err);
+pe_status count_blocks_launch(int n, const pe_slab* slabs, uint64_t* const* d_counts, void* desc, hipStream_t stream,
+                              std::string* err);
+size_t count_desc_bytes(int n);
 int64_t slab_blocks(const pe_slab& slab);
 pe_status launch_supertensor_format(bool decode, int filter, int turn, const void* in, int64_t nodes, void* out,
                                     hipStream_t stream, std::string* err);
@@ -78,20 +80,23 @@ pe_slab whole_slab(const pe_section& sec, const int shape[4], void* data) {
   return slab;
 }
 
-// Counts of n slabs summed per slab, through one device buffer and one copy back.
+// Counts of n slabs summed per slab: one launch over every block, one device buffer (per-block
+// counts, then the launch descriptors) and one copy back.
 pe_status count_slabs(int n, const pe_slab* slabs, uint64_t (*counts)[4]) {
   std::vector<int64_t> first(n + 1, 0);
   for (int k = 0; k < n; k++) first[k + 1] = first[k] + slab_blocks(slabs[k]);
   std::fill(&counts[0][0], &counts[0][0] + 4 * size_t(n), uint64_t(0));
   if (!first[n]) return PE_OK;
+  const size_t count_bytes = sizeof(uint64_t) * 4 * size_t(first[n]);
   void* d = nullptr;
-  if (hipMalloc(&d, sizeof(uint64_t) * 4 * first[n]) != hipSuccess) return failed(PE_ERR_DEVICE, "hipMalloc(counts) failed");
+  if (hipMalloc(&d, count_bytes + count_desc_bytes(n)) != hipSuccess)
+    return failed(PE_ERR_DEVICE, "hipMalloc(counts) failed");
+  std::vector<uint64_t*> dst(n);
+  for (int k = 0; k < n; k++) dst[k] = static_cast<uint64_t*>(d) + 4 * first[k];
   std::string err;
-  pe_status s = PE_OK;
-  for (int k = 0; k < n && s == PE_OK; k++)
-    s = count_slab_blocks(slabs[k], static_cast<uint64_t*>(d) + 4 * first[k], nullptr, &err);
+  pe_status s = count_blocks_launch(n, slabs, dst.data(), static_cast<char*>(d) + count_bytes, nullptr, &err);
   std::vector<uint64_t> h(4 * first[n]);
-  if (s == PE_OK && hipMemcpy(h.data(), d, sizeof(uint64_t) * h.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+  if (s == PE_OK && hipMemcpy(h.data(), d, count_bytes, hipMemcpyDeviceToHost) != hipSuccess) {
     s = PE_ERR_DEVICE;
     err = "counting blocks failed";
   }
@@ -120,9 +125,24 @@ extern "C" {
 pe_status pe_end_count_blocks(const pe_slab* slab, uint64_t* d_counts, void* stream) {
   if (!slab) return failed(PE_ERR_INVALID, "null slab");
   std::string err;
-  const pe_status s = count_slab_blocks(*slab, d_counts, static_cast<hipStream_t>(stream), &err);
+  const pe_status s = count_blocks_launch(1, slab, &d_counts, nullptr, static_cast<hipStream_t>(stream), &err);
   return s == PE_OK ? s : failed(s, err);
 }
+
+pe_status pe_end_count_blocks_many(int n, const pe_slab* slabs, uint64_t* d_counts, void* d_workspace, void* stream) {
+  if (n < 0 || (n && (!slabs || !d_counts || !d_workspace))) return failed(PE_ERR_INVALID, "bad count arguments");
+  std::vector<uint64_t*> dst(n);
+  int64_t first = 0;
+  for (int k = 0; k < n; k++) {
+    dst[k] = d_counts + 4 * first;
+    first += slab_blocks(slabs[k]);
+  }
+  std::string err;
+  const pe_status s = count_blocks_launch(n, slabs, dst.data(), d_workspace, static_cast<hipStream_t>(stream), &err);
+  return s == PE_OK ? s : failed(s, err);
+}
+
+int64_t pe_end_count_workspace_bytes(int n) { return int64_t(count_desc_bytes(n)); }
 
 pe_status pe_end_count_slabs(int n, const pe_slab* slabs, uint64_t (*counts)[4]) {
   if (n < 0 || (n && (!slabs || !counts))) return failed(PE_ERR_INVALID, "bad count arguments");

@@ -128,6 +128,38 @@ def test_count_blocks_seeded_against_oracle():
     assert np.array_equal(P.count_slabs([slab])[0], want.sum(axis=0))
 
 
+def test_count_blocks_many_matches_per_slab_launches():
+    # 150 slabs (more than 64, so the kernel's slab search takes two rounds): single blocks,
+    # partial boxes and whole small sections, each with its own seeded data
+    rng = np.random.default_rng(5)
+    secs = [((2, 0), (1, 1), (0, 1), (0, 1)), ((3, 4), (4, 4), (4, 4), (4, 3)), ((1, 2), (2, 1), (0, 0), (3, 3))]
+    slabs, bufs = [], []
+    for k in range(150):
+        counts = secs[k % 3]
+        nb = [(s + 7) // 8 for s in P.section_shape(counts)]
+        lo = [int(rng.integers(0, n)) for n in nb]
+        hi = [int(rng.integers(l + 1, min(n, l + 2) + 1)) for l, n in zip(lo, nb)]
+        blocks = int(np.prod([h - l for l, h in zip(lo, hi)]))
+        buf = hip.Buffer(64 * 4096 * blocks)
+        P.fill_seeded(buf.ptr, 4096 * blocks, 100 + k, 0)
+        bufs.append(buf)
+        slabs.append(P.Slab.of(counts, lo, hi, buf.ptr))
+    total = sum(s.blocks for s in slabs)
+    offs = np.cumsum([0] + [s.blocks for s in slabs])
+    one, many = hip.Buffer(32 * total), hip.Buffer(32 * total)
+    ws = hip.Buffer(P.count_workspace_bytes(len(slabs)))
+    for s, o in zip(slabs, offs):
+        P.count_blocks(s, one.ptr + 32 * int(o))
+    P.count_blocks_many(slabs, many.ptr, ws.ptr)
+    hip.synchronize()
+    a = one.download(np.zeros((total, 4), dtype=np.uint64))
+    b = many.download(np.zeros((total, 4), dtype=np.uint64))
+    assert np.array_equal(a, b)
+    assert int(a[:, 2].sum()) > 0
+    sums = np.array([a[offs[k]:offs[k + 1]].sum(axis=0) for k in range(len(slabs))], dtype=np.uint64)
+    assert np.array_equal(P.count_slabs(slabs), sums)
+
+
 def test_slice_35_matches_production_counts():
     s = store(35)
     counts = s.counts()
