@@ -54,13 +54,27 @@ PE_HD uint64_t shift2_word(uint64_t x, int k) {   // rotate 16-bit chunks within
 PE_HD S4 shift0(S4 a, int k) { for (int i = 0; i < 4; i++) a.w[i] = shift0_word(a.w[i], k & 3); return a; }
 PE_HD S4 shift1(S4 a, int k) { for (int i = 0; i < 4; i++) a.w[i] = shift1_word(a.w[i], k & 3); return a; }
 PE_HD S4 shift2(S4 a, int k) { for (int i = 0; i < 4; i++) a.w[i] = shift2_word(a.w[i], k & 3); return a; }
+// m ? b : a bitwise, m all ones or all zeros.  On the device one three-input logic
+// instruction per 32-bit half (truth-table index src0 * 4 + src1 * 2 + src2, 0xD8 = "src2 ?
+// src1 : src0"); written as ternaries or masked and/or the compiler emits three instructions
+// per half, or turns a chain of selects over the words into a private-memory array.
+PE_HD uint64_t select64(uint64_t a, uint64_t b, uint32_t m) {
+#ifdef __HIP_DEVICE_COMPILE__
+  const uint32_t lo = __builtin_amdgcn_bitop3_b32(uint32_t(a), uint32_t(b), m, 0xD8);
+  const uint32_t hi = __builtin_amdgcn_bitop3_b32(uint32_t(a >> 32), uint32_t(b >> 32), m, 0xD8);
+  return uint64_t(lo) | uint64_t(hi) << 32;
+#else
+  const uint64_t m64 = uint64_t(0) - uint64_t(m & 1);
+  return (a & ~m64) | (b & m64);
+#endif
+}
 PE_HD S4 shift3(S4 a, int k) {
-  // word rotation by k as two masked selects (a per-thread index into the words would be
+  // word rotation by k as two stages of selects (a per-thread index into the words would be
   // lowered to a private-memory array)
-  const uint64_t m1 = uint64_t(0) - uint64_t(k & 1), m2 = uint64_t(0) - uint64_t(k >> 1 & 1);
+  const uint32_t m1 = 0u - uint32_t(k & 1), m2 = 0u - uint32_t(k >> 1 & 1);
   S4 r, t;
-  for (int i = 0; i < 4; i++) r.w[i] = (a.w[i] & ~m1) | (a.w[(i + 1) & 3] & m1);
-  for (int i = 0; i < 4; i++) t.w[i] = (r.w[i] & ~m2) | (r.w[(i + 2) & 3] & m2);
+  for (int i = 0; i < 4; i++) r.w[i] = select64(a.w[i], a.w[(i + 1) & 3], m1);
+  for (int i = 0; i < 4; i++) t.w[i] = select64(r.w[i], r.w[(i + 2) & 3], m2);
   return t;
 }
 PE_HD S4 shift_coord_var(S4 a, int q, int k);
