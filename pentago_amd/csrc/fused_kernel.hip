@@ -21,8 +21,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <map>
+#include <string>
+#include <thread>
 #include <unordered_map>
 #include <tuple>
 #include <vector>
@@ -378,68 +381,104 @@ static pe_status pack_fused_plan(const std::vector<BlockDev>& blocks, const std:
   return PE_OK;
 }
 
+// One output slab's share of a fused plan, built independently of the others: its lines,
+// section record, field and move tables, and blocks (section 0, lines counted from 0).
+struct SlabBuild {
+  std::vector<LineHost> lines;
+  SecDev sec;
+  std::vector<uint64_t> fields;
+  std::vector<uint16_t> movetab;
+  std::vector<BlockDev> blocks;
+  int64_t nodes = 0;
+  pe_status status = PE_OK;
+  std::string err;
+};
+
+static void build_slab(const pe_slab& out, int nch, const pe_slab* ch, SlabBuild* r) {
+  const auto& Tb = tables();
+  if (!out.data) { r->err = "output slab without memory"; r->status = PE_ERR_INVALID; return; }
+  std::map<std::tuple<int, int, int, int>, int> index;
+  r->status = slab_lines(out, nch, ch, &r->lines, &index, &r->err);
+  if (r->status != PE_OK) return;
+  Section s;
+  std::memcpy(s.c, out.section.counts, 8);
+  if (s.sum() > 34) { r->err = "the fused kernel needs sections with at most 34 stones"; r->status = PE_ERR_INVALID; return; }
+  SecDev& sec = r->sec;
+  std::memset(&sec, 0, sizeof sec);
+  sec.turn = s.sum() & 1;
+  section_field_tables(s, &r->fields, sec.field_off);
+  r->status = section_move_tables(s, &r->movetab, sec.move_off, sec.nmoves, &r->err);
+  if (r->status != PE_OK) return;
+  for (int a = 0; a < 4; a++) {
+    const int b = rmin_bucket(s.c[a][0], s.c[a][1]);
+    sec.rmin_off[a] = Tb.rmin_offsets[b];
+    sec.rmin_size[a] = Tb.rmin_offsets[b + 1] - Tb.rmin_offsets[b];
+  }
+  int b[4];
+  for (b[0] = out.lo[0]; b[0] < out.hi[0]; b[0]++)
+    for (b[1] = out.lo[1]; b[1] < out.hi[1]; b[1]++)
+      for (b[2] = out.lo[2]; b[2] < out.hi[2]; b[2]++)
+        for (b[3] = out.lo[3]; b[3] < out.hi[3]; b[3]++) {
+          BlockDev B;
+          B.sec = 0;
+          B.accumulate = 0;
+          int n = 1;
+          for (int a = 0; a < 4; a++) { B.b[a] = b[a]; n *= std::min(8, sec.rmin_size[a] - 8 * b[a]); }
+          for (int d = 0; d < 4; d++) {
+            B.line[d] = -1;
+            if (s.c[d][0] + s.c[d][1] >= 9) continue;
+            int o[3], kk = 0;
+            for (int a = 0; a < 4; a++) if (a != d) o[kk++] = b[a];
+            B.line[d] = index.at(std::make_tuple(d, o[0], o[1], o[2]));
+          }
+          B.out = static_cast<uint64_t*>(out.data) + 8 * 4096 * slab_index(out, b);
+          r->blocks.push_back(B);
+          r->nodes += n;
+        }
+}
+
 pe_status fused_plan_create(int nout, const pe_slab* outs, int nch, const pe_slab* ch, FusedPlan* plan,
                             std::string* err) {
   DevTables T;
   if (device_tables(&T, err) != PE_OK) return PE_ERR_DEVICE;
-  const auto& Tb = tables();
+  // the slabs' shares are built on host threads (a slice has thousands of sections), then
+  // concatenated in slab order, so the plan does not depend on the thread count
+  std::vector<SlabBuild> builds(size_t(std::max(nout, 0)));
+  const int nthreads = std::max(1, std::min({nout, int(std::thread::hardware_concurrency()), 16}));
+  std::atomic<int> next(0);
+  auto work = [&]() {
+    for (int k; (k = next.fetch_add(1)) < nout;) build_slab(outs[k], nch, ch, &builds[size_t(k)]);
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nthreads; t++) pool.emplace_back(work);
+  work();
+  for (auto& th : pool) th.join();
   std::vector<LineHost> lines;
   std::vector<SecDev> secs;
   std::vector<uint64_t> fields;
   std::vector<uint16_t> movetab;
   std::vector<BlockDev> blocks;
   int64_t nodes = 0;
-  for (int k = 0; k < nout; k++) {
-    const pe_slab& out = outs[k];
-    if (!out.data) { *err = "output slab without memory"; return PE_ERR_INVALID; }
-    std::vector<LineHost> sl;
-    std::map<std::tuple<int, int, int, int>, int> index;
-    pe_status st = slab_lines(out, nch, ch, &sl, &index, err);
-    if (st != PE_OK) return st;
-    Section s;
-    std::memcpy(s.c, out.section.counts, 8);
-    if (s.sum() > 34) { *err = "the fused kernel needs sections with at most 34 stones"; return PE_ERR_INVALID; }
-    SecDev sec;
-    std::memset(&sec, 0, sizeof sec);
-    sec.turn = s.sum() & 1;
-    std::vector<uint64_t> f;
-    section_field_tables(s, &f, sec.field_off);
-    std::vector<uint16_t> mt;
-    st = section_move_tables(s, &mt, sec.move_off, sec.nmoves, err);
-    if (st != PE_OK) return st;
+  for (SlabBuild& r : builds) {
+    if (r.status != PE_OK) { *err = r.err; return r.status; }
+    SecDev sec = r.sec;
     for (int a = 0; a < 4; a++) {
       sec.field_off[a] += int32_t(fields.size() / 4);
       if (sec.move_off[a] >= 0) sec.move_off[a] += int32_t(movetab.size());
-      const int b = rmin_bucket(s.c[a][0], s.c[a][1]);
-      sec.rmin_off[a] = Tb.rmin_offsets[b];
-      sec.rmin_size[a] = Tb.rmin_offsets[b + 1] - Tb.rmin_offsets[b];
     }
-    fields.insert(fields.end(), f.begin(), f.end());
-    movetab.insert(movetab.end(), mt.begin(), mt.end());
+    fields.insert(fields.end(), r.fields.begin(), r.fields.end());
+    movetab.insert(movetab.end(), r.movetab.begin(), r.movetab.end());
     const int line0 = int(lines.size());
-    for (auto& h : sl) lines.push_back(std::move(h));
-    int b[4];
-    for (b[0] = out.lo[0]; b[0] < out.hi[0]; b[0]++)
-      for (b[1] = out.lo[1]; b[1] < out.hi[1]; b[1]++)
-        for (b[2] = out.lo[2]; b[2] < out.hi[2]; b[2]++)
-          for (b[3] = out.lo[3]; b[3] < out.hi[3]; b[3]++) {
-            BlockDev B;
-            B.sec = int32_t(secs.size());
-            B.accumulate = 0;
-            int n = 1;
-            for (int a = 0; a < 4; a++) { B.b[a] = b[a]; n *= std::min(8, sec.rmin_size[a] - 8 * b[a]); }
-            for (int d = 0; d < 4; d++) {
-              B.line[d] = -1;
-              if (s.c[d][0] + s.c[d][1] >= 9) continue;
-              int o[3], kk = 0;
-              for (int a = 0; a < 4; a++) if (a != d) o[kk++] = b[a];
-              B.line[d] = line0 + index.at(std::make_tuple(d, o[0], o[1], o[2]));
-            }
-            B.out = static_cast<uint64_t*>(out.data) + 8 * 4096 * slab_index(out, b);
-            blocks.push_back(B);
-            nodes += n;
-          }
+    for (auto& h : r.lines) lines.push_back(std::move(h));
+    for (BlockDev B : r.blocks) {
+      B.sec = int32_t(secs.size());
+      for (int d = 0; d < 4; d++)
+        if (B.line[d] >= 0) B.line[d] += line0;
+      blocks.push_back(B);
+    }
+    nodes += r.nodes;
     secs.push_back(sec);
+    std::vector<BlockDev>().swap(r.blocks);
   }
   return pack_fused_plan(blocks, lines, secs, fields, movetab, nodes, plan, err);
 }

@@ -360,11 +360,40 @@ pe_status pe_slice_plan_groups(int n, const pe_section* sections, int64_t budget
   return PE_OK;
 }
 
-pe_status pe_slice_grouped_counts(int n, const pe_section* sections, int64_t budget, const pe_slice* children,
-                                  const pe_slice* grandchildren, uint64_t (*counts)[4],
-                                  void (*visit)(void* ctx, const pe_slice* group), void* ctx, int* ngroups,
-                                  void* stream) {
-  if (!counts) return failed(PE_ERR_INVALID, "null counts");
+}  // extern "C"
+
+namespace {
+
+// Device bytes of the largest group: its output slabs, and its regenerated children.
+int64_t group_arena_bytes(const pe_section* sections, const std::vector<int>& ends, int ng, bool regen) {
+  int64_t bytes = 0;
+  for (int g = 0, begin = 0; g < ng; begin = ends[g++]) {
+    int64_t b = pe_slice_bytes(ends[g] - begin, sections + begin);
+    if (regen) {
+      const std::vector<pe_section> ks = group_kids(sections, begin, ends[g]);
+      b += pe_slice_bytes(int(ks.size()), ks.data());
+    }
+    bytes = std::max(bytes, b);
+  }
+  return bytes;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t pe_slice_arena_bytes(int n, const pe_section* sections, int64_t budget, int regenerate) {
+  std::vector<int> ends(std::max(n, 1));
+  int ng = 0;
+  if (pe_slice_plan_groups(n, sections, budget, regenerate, ends.data(), &ng) != PE_OK) return -1;
+  return group_arena_bytes(sections, ends, ng, regenerate != 0);
+}
+
+pe_status pe_slice_grouped_counts_in(int n, const pe_section* sections, int64_t budget, const pe_slice* children,
+                                     const pe_slice* grandchildren, uint64_t (*counts)[4],
+                                     void (*visit)(void* ctx, const pe_slice* group), void* ctx, int* ngroups,
+                                     void* arena, int64_t arena_bytes, void* stream) {
+  if (!counts || !arena) return failed(PE_ERR_INVALID, "null counts or arena");
   if (!children && !grandchildren) return failed(PE_ERR_INVALID, "need the next slice's store or the one after it");
   const bool regen = !children;
   std::vector<int> ends(std::max(n, 1));
@@ -372,17 +401,9 @@ pe_status pe_slice_grouped_counts(int n, const pe_section* sections, int64_t bud
   pe_status st = pe_slice_plan_groups(n, sections, budget, regen, ends.data(), &ng);
   if (st != PE_OK) return st;
   if (ngroups) *ngroups = ng;
-  // one arena for every group (its outputs, then its regenerated children): device memory
-  // is allocated once, not per section and group
-  int64_t arena_bytes = 0;
-  for (int g = 0, begin = 0; g < ng; begin = ends[g++]) {
-    int64_t b = pe_slice_bytes(ends[g] - begin, sections + begin);
-    if (regen) b += pe_slice_bytes(int(group_kids(sections, begin, ends[g]).size()),
-                                   group_kids(sections, begin, ends[g]).data());
-    arena_bytes = std::max(arena_bytes, b);
-  }
-  void* arena = nullptr;
-  if (hipMalloc(&arena, size_t(arena_bytes)) != hipSuccess) return failed(PE_ERR_CAPACITY, "hipMalloc(group arena) failed");
+  if (group_arena_bytes(sections, ends, ng, regen) > arena_bytes)
+    return failed(PE_ERR_CAPACITY, "the arena is smaller than pe_slice_arena_bytes");
+  // every group (its outputs, then its regenerated children) is carved from the one arena
   for (int g = 0, begin = 0; g < ng; begin = ends[g++]) {
     pe_slice* kids = nullptr;
     const int64_t out_bytes = pe_slice_bytes(ends[g] - begin, sections + begin);
@@ -401,6 +422,22 @@ pe_status pe_slice_grouped_counts(int n, const pe_section* sections, int64_t bud
     pe_slice_destroy(kids);
     if (st != PE_OK) break;
   }
+  return st;
+}
+
+pe_status pe_slice_grouped_counts(int n, const pe_section* sections, int64_t budget, const pe_slice* children,
+                                  const pe_slice* grandchildren, uint64_t (*counts)[4],
+                                  void (*visit)(void* ctx, const pe_slice* group), void* ctx, int* ngroups,
+                                  void* stream) {
+  if (!counts) return failed(PE_ERR_INVALID, "null counts");
+  if (!children && !grandchildren) return failed(PE_ERR_INVALID, "need the next slice's store or the one after it");
+  const int64_t bytes = pe_slice_arena_bytes(n, sections, budget, !children);
+  if (bytes < 0) return failed(PE_ERR_INVALID, "invalid sections or budget");
+  void* arena = nullptr;
+  if (hipMalloc(&arena, size_t(std::max<int64_t>(bytes, 1))) != hipSuccess)
+    return failed(PE_ERR_CAPACITY, "hipMalloc(group arena) failed");
+  const pe_status st = pe_slice_grouped_counts_in(n, sections, budget, children, grandchildren, counts, visit, ctx,
+                                                  ngroups, arena, bytes, stream);
   (void)hipFree(arena);
   return st;
 }

@@ -41,6 +41,15 @@ def plan_groups(sections, budget, regenerate):
     return out
 
 
+def arena_bytes(sections, budget, regenerate):
+    """Device bytes grouped_counts needs for its largest group (host only)."""
+    arr = (Section * len(sections))(*[Section.of(c) for c in sections])
+    b = int(lib().pe_slice_arena_bytes(len(sections), arr, int(budget), int(bool(regenerate))))
+    if b < 0:
+        raise ValueError("invalid sections or budget")
+    return b
+
+
 class GroupView:
     """A computed group inside grouped_counts (valid during the visit only)."""
 
@@ -62,14 +71,16 @@ class GroupView:
 VISIT = C.CFUNCTYPE(None, C.c_void_p, C.c_void_p)
 
 
-def grouped_counts(sections, budget, children=None, grandchildren=None, visit=None, stream=None):
+def grouped_counts(sections, budget, children=None, grandchildren=None, visit=None, stream=None, arena=None):
     """Counts of every section of one slice, computed in groups within `budget` bytes by the
     native slice loop (pe_slice_grouped_counts).
 
     children: the whole store of the next slice, or None with grandchildren: the whole store
     of the slice after it, from which each group's child sections are regenerated.
-    visit(group) sees each computed group (a GroupView) before it is freed.  Returns
-    (sections in order, (n, 4) counts)."""
+    visit(group) sees each computed group (a GroupView) before it is freed.  arena: device
+    memory (an object with .ptr and .nbytes, at least arena_bytes(...)) to compute in, reused
+    across calls; None allocates one for the call.  Returns (sections in order, (n, 4)
+    counts)."""
     if children is None and grandchildren is None:
         raise ValueError("need the next slice's store or the one after it")
     sections = [tuple(tuple(x) for x in s) for s in sections]
@@ -85,9 +96,15 @@ def grouped_counts(sections, budget, children=None, grandchildren=None, visit=No
 
     fn = VISIT(cb) if visit else VISIT()
     ng = C.c_int()
-    _check(lib().pe_slice_grouped_counts(len(sections), arr, int(budget), children.handle if children else None,
-                                         grandchildren.handle if grandchildren else None, counts.ctypes.data,
-                                         fn, None, C.byref(ng), stream), "pe_slice_grouped_counts")
+    ch = children.handle if children else None
+    gch = grandchildren.handle if grandchildren else None
+    if arena is None:
+        _check(lib().pe_slice_grouped_counts(len(sections), arr, int(budget), ch, gch, counts.ctypes.data,
+                                             fn, None, C.byref(ng), stream), "pe_slice_grouped_counts")
+    else:
+        _check(lib().pe_slice_grouped_counts_in(len(sections), arr, int(budget), ch, gch, counts.ctypes.data,
+                                                fn, None, C.byref(ng), arena.ptr, int(arena.nbytes), stream),
+               "pe_slice_grouped_counts_in")
     if errors:
         raise errors[0]
     return sections, counts

@@ -30,6 +30,17 @@ int main(void) {
   int ends[330], groups = 0;
   CHECK(pe_slice_plan_groups(n, secs, (int64_t)4 << 30, 0, ends, &groups) == PE_OK && groups > 1 &&
             ends[groups - 1] == n, "group planning");
+  {
+    /* the arena holds the largest group: at most the budget, at least its largest section */
+    int64_t largest = 0;
+    for (int k = 0; k < n; k++) {
+      const int64_t b = pe_slice_bytes(1, &secs[k]);
+      if (b > largest) largest = b;
+    }
+    const int64_t arena = pe_slice_arena_bytes(n, secs, (int64_t)4 << 30, 0);
+    CHECK(arena >= largest && arena <= ((int64_t)4 << 30), "arena bytes %lld", (long long)arena);
+    CHECK(pe_slice_arena_bytes(n, secs, 0, 0) == -1, "a zero budget is rejected");
+  }
 
   pe_line line;
   memset(&line, 0, sizeof line);

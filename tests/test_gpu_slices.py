@@ -202,7 +202,14 @@ def test_slice_32_matches_production_counts():
     secs = P.descendent_sections(EMPTY, 32)
     free, _ = hip.mem_info()
     budget = free - 16e9
-    done, counts = S.grouped_counts(secs, budget, grandchildren=grandchildren)
+    # in a caller arena (pe_slice_grouped_counts_in); one byte short is refused
+    need = S.arena_bytes(secs, budget, True)
+    assert 0 < need <= budget
+    arena = hip.Buffer(need)
+    with pytest.raises(P.PentagoError):
+        S.grouped_counts(secs, budget, grandchildren=grandchildren, arena=type("A", (), {"ptr": arena.ptr, "nbytes": need - 1})())
+    done, counts = S.grouped_counts(secs, budget, grandchildren=grandchildren, arena=arena)
+    del arena
     release(34)
     assert int(counts[:, 3].sum()) == 0
     assert np.array_equal(P.counts_rows(done, counts, 32), golden_rows(32))
