@@ -215,7 +215,8 @@ int pe_slice_sections(const pe_slice* s);
 pe_status pe_slice_counts(const pe_slice* s, uint64_t (*counts)[4]);
 pe_status pe_slice_destroy(pe_slice* s);
 /* The slice loop for a slice too large to keep whole (pentago/mpi/toplevel.cc:263-421 on one
- * device): sections[0..n) of one slice are computed in groups, in order, each group's slabs
+ * device): sections[0..n) of one slice are computed in the groups of pe_slice_plan_groups,
+ * each group's slabs
  * (and its regenerated children) within `budget` bytes.  Children come from `children`, the
  * whole store of the next slice, or -- when it is NULL -- are regenerated per group from
  * `grandchildren`, the whole store of the slice after it.  counts[k] = pe_slice_counts of
@@ -232,10 +233,14 @@ pe_status pe_slice_grouped_counts_in(int n, const pe_section* sections, int64_t 
                                      const pe_slice* grandchildren, uint64_t (*counts)[4],
                                      void (*visit)(void* ctx, const pe_slice* group), void* ctx, int* groups,
                                      void* arena, int64_t arena_bytes, void* stream);
-/* The groups pe_slice_grouped_counts uses (host only): group g is sections
- * [ends[g-1], ends[g]) with ends[-1] = 0; ends needs room for n entries. */
-pe_status pe_slice_plan_groups(int n, const pe_section* sections, int64_t budget, int regenerate, int* ends,
-                               int* ngroups);
+/* The groups pe_slice_grouped_counts uses (host only): the sections are taken in the order
+ * sections[order[0]], sections[order[1]], ... and group g is positions [ends[g-1], ends[g])
+ * of that order, ends[-1] = 0; order and ends need room for n entries.  Without
+ * regeneration the order is the given one and each group as long as it fits; with it,
+ * sections that share child sections are grouped together (each child is regenerated once
+ * per group that needs it). */
+pe_status pe_slice_plan_groups(int n, const pe_section* sections, int64_t budget, int regenerate, int* order,
+                               int* ends, int* ngroups);
 /* Device bytes of the largest of those groups (outputs plus regenerated children), -1 on
  * invalid arguments (host only). */
 int64_t pe_slice_arena_bytes(int n, const pe_section* sections, int64_t budget, int regenerate);

@@ -167,7 +167,8 @@ def lib():
             "pe_slice_sections": (C.c_int, [V]),
             "pe_slice_counts": (C.c_int, [V, V]),
             "pe_slice_destroy": (C.c_int, [V]),
-            "pe_slice_plan_groups": (C.c_int, [C.c_int, P(Section), C.c_int64, C.c_int, P(C.c_int), P(C.c_int)]),
+            "pe_slice_plan_groups": (C.c_int, [C.c_int, P(Section), C.c_int64, C.c_int, P(C.c_int), P(C.c_int),
+                                               P(C.c_int)]),
             "pe_slice_grouped_counts": (C.c_int, [C.c_int, P(Section), C.c_int64, V, V, V,
                                                   C.CFUNCTYPE(None, V, V), V, P(C.c_int), V]),
             "pe_slice_grouped_counts_in": (C.c_int, [C.c_int, P(Section), C.c_int64, V, V, V,

@@ -10,6 +10,8 @@
 #include <algorithm>
 #include <cstring>
 #include <set>
+#include <unordered_map>
+#include <unordered_set>
 #include <string>
 #include <vector>
 
@@ -315,45 +317,95 @@ pe_status pe_slice_counts(const pe_slice* s, uint64_t (*counts)[4]) {
   return count_slabs(int(s->slabs.size()), s->slabs.data(), counts);
 }
 
-pe_status pe_slice_plan_groups(int n, const pe_section* sections, int64_t budget, int regenerate, int* ends,
-                               int* ngroups) {
-  if (n <= 0 || !sections || !ends || !ngroups || budget <= 0) return failed(PE_ERR_INVALID, "bad group arguments");
-  auto bytes_of = [](const Section& s) {
-    pe_section p;
-    std::memcpy(p.counts, s.c, 8);
-    return pe_slice_bytes(1, &p);
-  };
-  int count = 0;
-  int64_t used = 0;
-  std::set<uint64_t> kids;
+pe_status pe_slice_plan_groups(int n, const pe_section* sections, int64_t budget, int regenerate, int* order,
+                               int* ends, int* ngroups) {
+  if (n <= 0 || !sections || !order || !ends || !ngroups || budget <= 0)
+    return failed(PE_ERR_INVALID, "bad group arguments");
+  std::vector<int64_t> own(n);
+  std::vector<std::vector<uint64_t>> kids(n);   // standardized child sections (regeneration only)
+  std::unordered_map<uint64_t, int64_t> kid_bytes;
   for (int k = 0; k < n; k++) {
     Section s;
     int shape[4];
     if (!section_of(sections[k], &s, shape)) return failed(PE_ERR_INVALID, "invalid section " + std::to_string(k));
-    std::set<uint64_t> all, fresh;
-    if (regenerate)
-      for (int q = 0; q < 4; q++)
-        if (s.c[q][0] + s.c[q][1] < 9) {
-          Section t;
-          s.child(q).standardize(&t);
-          all.insert(t.sig());
+    own[k] = pe_slice_bytes(1, &sections[k]);
+    if (!regenerate) continue;
+    for (int q = 0; q < 4; q++)
+      if (s.c[q][0] + s.c[q][1] < 9) {
+        Section t;
+        s.child(q).standardize(&t);
+        const uint64_t sig = t.sig();
+        if (std::find(kids[k].begin(), kids[k].end(), sig) != kids[k].end()) continue;
+        kids[k].push_back(sig);
+        if (!kid_bytes.count(sig)) {
+          pe_section p;
+          std::memcpy(p.counts, t.c, 8);
+          kid_bytes[sig] = pe_slice_bytes(1, &p);
         }
-    auto kid_bytes = [&](const std::set<uint64_t>& sigs) {
-      int64_t b = 0;
-      for (uint64_t sig : sigs) b += bytes_of(section_from_sig(sig));
-      return b;
-    };
-    for (uint64_t sig : all)
-      if (!kids.count(sig)) fresh.insert(sig);
-    const int64_t need = bytes_of(s);
-    if (k > 0 && used + need + kid_bytes(fresh) > budget) {
-      ends[count++] = k;
-      used = 0;
-      kids.clear();
-      fresh = all;
+      }
+  }
+  int count = 0;
+  if (!regenerate) {   // sections in the given order, each group as long as it fits
+    int64_t used = 0;
+    for (int k = 0; k < n; k++) {
+      order[k] = k;
+      if (k > 0 && used + own[k] > budget) {
+        ends[count++] = k;
+        used = 0;
+      }
+      used += own[k];
     }
-    used += need + kid_bytes(fresh);
-    kids.insert(fresh.begin(), fresh.end());
+    ends[count++] = n;
+    *ngroups = count;
+    return PE_OK;
+  }
+  // With regeneration a group pays for its sections' children once, so sections that share
+  // children go together: each group is filled by repeatedly taking the remaining section
+  // that fits and adds the fewest new child bytes net of the child bytes it shares (ties:
+  // the larger section, then the earlier one), and is closed when none fits.
+  // fresh[k] / shared[k]: child bytes of section k outside / inside the current group, kept
+  // up to date through the sections that use each child
+  std::unordered_map<uint64_t, std::vector<int>> users;
+  std::vector<int64_t> fresh(n, 0), shared(n, 0);
+  for (int k = 0; k < n; k++)
+    for (uint64_t sig : kids[k]) {
+      users[sig].push_back(k);
+      fresh[k] += kid_bytes[sig];
+    }
+  std::vector<char> taken(n, 0);
+  std::unordered_set<uint64_t> group;
+  int placed = 0, in_group = 0;
+  int64_t used = 0;
+  while (placed < n) {
+    int best = -1;
+    for (int k = 0; k < n; k++) {
+      if (taken[k] || (in_group && used + own[k] + fresh[k] > budget)) continue;
+      const int64_t score = shared[k] - fresh[k];
+      const int64_t best_score = best < 0 ? 0 : shared[best] - fresh[best];
+      if (best < 0 || score > best_score || (score == best_score && own[k] > own[best])) best = k;
+    }
+    if (best < 0) {   // nothing fits: close the group
+      ends[count++] = placed;
+      for (uint64_t sig : group)
+        for (int u : users[sig]) {
+          fresh[u] += kid_bytes[sig];
+          shared[u] -= kid_bytes[sig];
+        }
+      group.clear();
+      used = 0;
+      in_group = 0;
+      continue;
+    }
+    taken[best] = 1;
+    order[placed++] = best;
+    in_group++;
+    used += own[best] + fresh[best];
+    for (uint64_t sig : kids[best])
+      if (group.insert(sig).second)
+        for (int u : users[sig]) {
+          fresh[u] -= kid_bytes[sig];
+          shared[u] += kid_bytes[sig];
+        }
   }
   ends[count++] = n;
   *ngroups = count;
@@ -383,10 +435,12 @@ int64_t group_arena_bytes(const pe_section* sections, const std::vector<int>& en
 extern "C" {
 
 int64_t pe_slice_arena_bytes(int n, const pe_section* sections, int64_t budget, int regenerate) {
-  std::vector<int> ends(std::max(n, 1));
+  std::vector<int> order(std::max(n, 1)), ends(std::max(n, 1));
   int ng = 0;
-  if (pe_slice_plan_groups(n, sections, budget, regenerate, ends.data(), &ng) != PE_OK) return -1;
-  return group_arena_bytes(sections, ends, ng, regenerate != 0);
+  if (pe_slice_plan_groups(n, sections, budget, regenerate, order.data(), ends.data(), &ng) != PE_OK) return -1;
+  std::vector<pe_section> ps(n);
+  for (int k = 0; k < n; k++) ps[k] = sections[order[k]];
+  return group_arena_bytes(ps.data(), ends, ng, regenerate != 0);
 }
 
 pe_status pe_slice_grouped_counts_in(int n, const pe_section* sections, int64_t budget, const pe_slice* children,
@@ -396,13 +450,19 @@ pe_status pe_slice_grouped_counts_in(int n, const pe_section* sections, int64_t 
   if (!counts || !arena) return failed(PE_ERR_INVALID, "null counts or arena");
   if (!children && !grandchildren) return failed(PE_ERR_INVALID, "need the next slice's store or the one after it");
   const bool regen = !children;
-  std::vector<int> ends(std::max(n, 1));
+  std::vector<int> order(std::max(n, 1)), ends(std::max(n, 1));
   int ng = 0;
-  pe_status st = pe_slice_plan_groups(n, sections, budget, regen, ends.data(), &ng);
+  pe_status st = pe_slice_plan_groups(n, sections, budget, regen, order.data(), ends.data(), &ng);
   if (st != PE_OK) return st;
   if (ngroups) *ngroups = ng;
-  if (group_arena_bytes(sections, ends, ng, regen) > arena_bytes)
+  // the sections in plan order; counts go back to the caller's order at the end
+  std::vector<pe_section> ps(n);
+  for (int k = 0; k < n; k++) ps[k] = sections[order[k]];
+  if (group_arena_bytes(ps.data(), ends, ng, regen) > arena_bytes)
     return failed(PE_ERR_CAPACITY, "the arena is smaller than pe_slice_arena_bytes");
+  std::vector<uint64_t> pc(4 * size_t(n), 0);
+  auto* pcounts = reinterpret_cast<uint64_t(*)[4]>(pc.data());
+  sections = ps.data();
   // every group (its outputs, then its regenerated children) is carved from the one arena
   for (int g = 0, begin = 0; g < ng; begin = ends[g++]) {
     pe_slice* kids = nullptr;
@@ -416,12 +476,15 @@ pe_status pe_slice_grouped_counts_in(int n, const pe_section* sections, int64_t 
     pe_slice* out = nullptr;
     if (st == PE_OK) st = pe_slice_create_in(ends[g] - begin, sections + begin, arena, out_bytes, &out);
     if (st == PE_OK) st = pe_slice_compute(out, regen ? kids : children, stream);
-    if (st == PE_OK) st = pe_slice_counts(out, counts + begin);
+    if (st == PE_OK) st = pe_slice_counts(out, pcounts + begin);
     if (st == PE_OK && visit) visit(ctx, out);
     pe_slice_destroy(out);
     pe_slice_destroy(kids);
     if (st != PE_OK) break;
   }
+  if (st == PE_OK)
+    for (int k = 0; k < n; k++)
+      for (int j = 0; j < 4; j++) counts[order[k]][j] = pcounts[k][j];
   return st;
 }
 

@@ -28,15 +28,17 @@ def child_sections(section):
 
 
 def plan_groups(sections, budget, regenerate):
-    """The groups grouped_counts computes (host only): list of section lists, in order."""
+    """The groups grouped_counts computes (host only): list of section lists, in the order
+    they are computed (with regeneration, sections sharing child sections go together)."""
     arr = (Section * len(sections))(*[Section.of(c) for c in sections])
+    order = (C.c_int * len(sections))()
     ends = (C.c_int * len(sections))()
     n = C.c_int()
-    _check(lib().pe_slice_plan_groups(len(sections), arr, int(budget), int(bool(regenerate)), ends, C.byref(n)),
-           "pe_slice_plan_groups")
+    _check(lib().pe_slice_plan_groups(len(sections), arr, int(budget), int(bool(regenerate)), order, ends,
+                                      C.byref(n)), "pe_slice_plan_groups")
     out, begin = [], 0
     for g in range(n.value):
-        out.append(list(sections[begin:ends[g]]))
+        out.append([sections[order[k]] for k in range(begin, ends[g])])
         begin = ends[g]
     return out
 

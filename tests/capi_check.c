@@ -27,9 +27,9 @@ int main(void) {
   printf("slice 35: %d sections, %lld slab bytes\n", n, (long long)bytes);
   CHECK(bytes == 60354LL * 4096 * 64, "slice 35 slab bytes %lld (60354 blocks of 4096 nodes)", (long long)bytes);
 
-  int ends[330], groups = 0;
-  CHECK(pe_slice_plan_groups(n, secs, (int64_t)4 << 30, 0, ends, &groups) == PE_OK && groups > 1 &&
-            ends[groups - 1] == n, "group planning");
+  int order[330], ends[330], groups = 0;
+  CHECK(pe_slice_plan_groups(n, secs, (int64_t)4 << 30, 0, order, ends, &groups) == PE_OK && groups > 1 &&
+            ends[groups - 1] == n && order[0] == 0 && order[n - 1] == n - 1, "group planning");
   {
     /* the arena holds the largest group: at most the budget, at least its largest section */
     int64_t largest = 0;

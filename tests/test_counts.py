@@ -154,14 +154,30 @@ def test_slice_driver_grouping():
     assert sorted(kids) == sorted(P.descendent_sections(root, 33))
     budget = 150e9
     gs = S.plan_groups(s32, budget, True)
-    assert [s for g in gs for s in g] == s32 and len(gs) > 1
+    assert sorted(s for g in gs for s in g) == sorted(s32) and len(gs) > 1
+    kids = {s: S.child_sections(s) for s in s32}
+    own = {s: P.slice_bytes([s]) for s in s32}
+    kb = {c: P.slice_bytes([c]) for s in s32 for c in kids[s]}
+
+    def need(g):
+        return sum(own[s] for s in g) + sum(kb[c] for c in {c for s in g for c in kids[s]})
+
+    later = set(s32)
     for g in gs:
-        need = P.slice_bytes(g) + P.slice_bytes(sorted({c for s in g for c in S.child_sections(s)}))
-        assert need <= budget or len(g) == 1
-        # greedy: the next group's first section would not have fit
-    for g, h in zip(gs, gs[1:]):
-        more = g + h[:1]
-        assert P.slice_bytes(more) + P.slice_bytes(sorted({c for s in more for c in S.child_sections(s)})) > budget
+        assert need(g) <= budget or len(g) == 1
+        later -= set(g)
+        # a group is closed only when no remaining section fits in it
+        assert all(need(g + [s]) > budget for s in later)
+    # sharing children: fewer child bytes regenerated than taking the sections in order
+    regen = sum(need(g) - sum(own[s] for s in g) for g in gs)
+    seq, cur = 0, []
+    for s in s32:
+        if cur and need(cur + [s]) > budget:
+            seq += need(cur) - sum(own[x] for x in cur)
+            cur = []
+        cur.append(s)
+    seq += need(cur) - sum(own[x] for x in cur)
+    assert regen < seq
     # without regeneration only the sections' own slabs count
     gs = S.plan_groups(s32, budget, False)
     assert all(P.slice_bytes(g) <= budget for g in gs)
