@@ -365,7 +365,14 @@ static pe_status pack_fused_plan(const std::vector<BlockDev>& blocks, const std:
   if (!movetab.empty()) std::memcpy(blob.data() + o_moves, movetab.data(), sizeof(uint16_t) * movetab.size());
   std::memcpy(blob.data() + o_offs, offs.data(), sizeof(int64_t) * offs.size());
   std::memcpy(blob.data() + o_refl, refl.data(), refl.size());
-  if (hipMemcpy(d, blob.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+  // uploaded on a stream of its own that does not wait for other work, so that a plan can be
+  // built while another plan's kernels run
+  hipStream_t ups = nullptr;
+  const bool copied = hipStreamCreateWithFlags(&ups, hipStreamNonBlocking) == hipSuccess &&
+                      hipMemcpyAsync(d, blob.data(), bytes, hipMemcpyHostToDevice, ups) == hipSuccess &&
+                      hipStreamSynchronize(ups) == hipSuccess;
+  if (ups) (void)hipStreamDestroy(ups);
+  if (!copied) {
     (void)hipFree(d);
     *err = "hipMemcpy(fused plan) failed";
     return PE_ERR_DEVICE;

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <future>
 #include <set>
 #include <unordered_map>
 #include <unordered_set>
@@ -463,24 +464,67 @@ pe_status pe_slice_grouped_counts_in(int n, const pe_section* sections, int64_t 
   std::vector<uint64_t> pc(4 * size_t(n), 0);
   auto* pcounts = reinterpret_cast<uint64_t(*)[4]>(pc.data());
   sections = ps.data();
-  // every group (its outputs, then its regenerated children) is carved from the one arena
-  for (int g = 0, begin = 0; g < ng; begin = ends[g++]) {
+  // A group's stores are carved from the one arena (its outputs, then its regenerated
+  // children) and its plans are built on a host thread while the previous group's kernels
+  // run: building a plan reads addresses only, never the memory behind them.
+  struct Group {
     pe_slice* kids = nullptr;
+    pe_slice* out = nullptr;
+    pe_end_plan* kid_plan = nullptr;
+    pe_end_plan* out_plan = nullptr;
+    pe_status st = PE_OK;
+    std::string err;
+    void release() {
+      pe_end_plan_destroy(kid_plan);
+      pe_end_plan_destroy(out_plan);
+      pe_slice_destroy(out);
+      pe_slice_destroy(kids);
+      kid_plan = out_plan = nullptr;
+      out = kids = nullptr;
+    }
+  };
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  auto prepare = [&, dev](int g) {
+    Group G;
+    (void)hipSetDevice(dev);
+    const int begin = g ? ends[g - 1] : 0;
     const int64_t out_bytes = pe_slice_bytes(ends[g] - begin, sections + begin);
     if (regen) {
       const std::vector<pe_section> ks = group_kids(sections, begin, ends[g]);
-      st = pe_slice_create_in(int(ks.size()), ks.data(), static_cast<uint8_t*>(arena) + out_bytes,
-                              arena_bytes - out_bytes, &kids);
-      if (st == PE_OK) st = pe_slice_compute(kids, grandchildren, stream);
+      G.st = pe_slice_create_in(int(ks.size()), ks.data(), static_cast<uint8_t*>(arena) + out_bytes,
+                                arena_bytes - out_bytes, &G.kids);
+      if (G.st == PE_OK)
+        G.st = pe_end_slabs_plan_create(int(G.kids->slabs.size()), G.kids->slabs.data(),
+                                        int(grandchildren->slabs.size()), grandchildren->slabs.data(), 0,
+                                        &G.kid_plan);
     }
-    pe_slice* out = nullptr;
-    if (st == PE_OK) st = pe_slice_create_in(ends[g] - begin, sections + begin, arena, out_bytes, &out);
-    if (st == PE_OK) st = pe_slice_compute(out, regen ? kids : children, stream);
-    if (st == PE_OK) st = pe_slice_counts(out, pcounts + begin);
-    if (st == PE_OK && visit) visit(ctx, out);
-    pe_slice_destroy(out);
-    pe_slice_destroy(kids);
+    if (G.st == PE_OK) G.st = pe_slice_create_in(ends[g] - begin, sections + begin, arena, out_bytes, &G.out);
+    const pe_slice* ch = regen ? G.kids : children;
+    if (G.st == PE_OK)
+      G.st = pe_end_slabs_plan_create(int(G.out->slabs.size()), G.out->slabs.data(), int(ch->slabs.size()),
+                                      ch->slabs.data(), 0, &G.out_plan);
+    if (G.st != PE_OK) G.err = pe_last_error();
+    return G;
+  };
+  std::future<Group> next = std::async(std::launch::async, prepare, 0);
+  for (int g = 0, begin = 0; g < ng; begin = ends[g++]) {
+    Group G = next.get();
+    if (G.st == PE_OK && g + 1 < ng) next = std::async(std::launch::async, prepare, g + 1);
+    st = G.st;
+    if (st != PE_OK) set_error(G.err);
+    if (st == PE_OK && G.kid_plan) st = pe_end_plan_run(G.kid_plan, stream);
+    if (st == PE_OK) st = pe_end_plan_run(G.out_plan, stream);
+    if (st == PE_OK && hipStreamSynchronize(static_cast<hipStream_t>(stream)) != hipSuccess)
+      st = failed(PE_ERR_DEVICE, "group compute failed");
+    if (st == PE_OK) st = pe_slice_counts(G.out, pcounts + begin);
+    if (st == PE_OK && visit) visit(ctx, G.out);
+    G.release();
     if (st != PE_OK) break;
+  }
+  if (next.valid()) {   // a group prepared after a failure: release it
+    Group G = next.get();
+    G.release();
   }
   if (st == PE_OK)
     for (int k = 0; k < n; k++)
