@@ -117,6 +117,7 @@ using namespace pe;
 extern "C" {
 
 pe_status pe_init(void) {
+  PE_GUARD_BEGIN
   try {
     tables();
   } catch (const std::exception& e) {
@@ -127,11 +128,13 @@ pe_status pe_init(void) {
   const pe_status s = device_tables(&D, &err);
   if (s != PE_OK) return fail(s, err);
   return PE_OK;
+  PE_GUARD_END
 }
 
 const char* pe_last_error(void) { return g_error.c_str(); }
 
 pe_status pe_table_digest(const char* name, char out[41]) {
+  PE_GUARD_BEGIN
   try {
     const std::string d = table_digest(name);
     if (d.empty()) return fail(PE_ERR_INVALID, std::string("unknown table ") + name);
@@ -140,9 +143,11 @@ pe_status pe_table_digest(const char* name, char out[41]) {
   } catch (const std::exception& e) {
     return fail(PE_ERR_TABLES, e.what());
   }
+  PE_GUARD_END
 }
 
 pe_status pe_section_shape(const pe_section* section, int32_t shape[4]) {
+  PE_GUARD_BEGIN
   if (!section || !shape) return fail(PE_ERR_INVALID, "null argument");
   Section s;
   std::memcpy(s.c, section->counts, sizeof s.c);
@@ -150,9 +155,11 @@ pe_status pe_section_shape(const pe_section* section, int32_t shape[4]) {
   if (!s.valid_counts() || !s.shape(sh)) return fail(PE_ERR_INVALID, "invalid section");
   for (int q = 0; q < 4; q++) shape[q] = sh[q];
   return PE_OK;
+  PE_GUARD_END
 }
 
 pe_status pe_section_standardize(const pe_section* section, pe_section* standard, int32_t* symmetry) {
+  PE_GUARD_BEGIN
   if (!section || !standard) return fail(PE_ERR_INVALID, "null argument");
   Section s, t;
   std::memcpy(s.c, section->counts, sizeof s.c);
@@ -161,9 +168,11 @@ pe_status pe_section_standardize(const pe_section* section, pe_section* standard
   std::memcpy(standard->counts, t.c, sizeof t.c);
   if (symmetry) *symmetry = g;
   return PE_OK;
+  PE_GUARD_END
 }
 
 pe_status pe_line_describe(const pe_line* line, pe_line_info* info) {
+  PE_GUARD_BEGIN
   if (!line || !info) return fail(PE_ERR_INVALID, "null argument");
   LineHost h;
   std::string err;
@@ -171,10 +180,12 @@ pe_status pe_line_describe(const pe_line* line, pe_line_info* info) {
   if (s != PE_OK) return fail(s, err);
   *info = h.info;
   return PE_OK;
+  PE_GUARD_END
 }
 
 pe_status pe_end_compute_lines(int n, const pe_line* lines, const void* const* d_inputs, void* const* d_outputs,
                                void* stream) {
+  PE_GUARD_BEGIN
   if (n < 0 || (n && (!lines || !d_inputs || !d_outputs))) return fail(PE_ERR_INVALID, "bad line batch");
   std::vector<pe_line_job> jobs(n);
   for (int k = 0; k < n; k++) {
@@ -192,6 +203,7 @@ pe_status pe_end_compute_lines(int n, const pe_line* lines, const void* const* d
     s = fail(PE_ERR_DEVICE, "line compute failed");
   pe_end_plan_destroy(plan);
   return s;
+  PE_GUARD_END
 }
 
 struct pe_end_plan {
@@ -201,6 +213,7 @@ struct pe_end_plan {
 };
 
 pe_status pe_end_plan_create(int n, const pe_line_job* jobs, pe_end_plan** out) {
+  PE_GUARD_BEGIN
   if (n < 0 || !out || (n && !jobs)) return fail(PE_ERR_INVALID, "bad plan arguments");
   std::vector<LineHost> hosts(n);
   for (int k = 0; k < n; k++) {
@@ -245,15 +258,19 @@ pe_status pe_end_plan_create(int n, const pe_line_job* jobs, pe_end_plan** out) 
   }
   *out = plan;
   return PE_OK;
+  PE_GUARD_END
 }
 
 pe_status pe_end_slab_plan_create(const pe_slab* output, int nchildren, const pe_slab* children, int mode,
                                   pe_end_plan** out) {
+  PE_GUARD_BEGIN
   return pe_end_slabs_plan_create(1, output, nchildren, children, mode, out);
+  PE_GUARD_END
 }
 
 pe_status pe_end_slabs_plan_create(int noutputs, const pe_slab* outputs, int nchildren, const pe_slab* children,
                                    int mode, pe_end_plan** out) {
+  PE_GUARD_BEGIN
   if (noutputs <= 0 || !outputs || !out || nchildren < 0 || (nchildren && !children))
     return fail(PE_ERR_INVALID, "bad slab plan arguments");
   for (int k = 0; k < noutputs; k++)
@@ -294,9 +311,11 @@ pe_status pe_end_slabs_plan_create(int noutputs, const pe_slab* outputs, int nch
   }
   *out = plan;
   return PE_OK;
+  PE_GUARD_END
 }
 
 pe_status pe_end_plan_run(const pe_end_plan* plan, void* stream) {
+  PE_GUARD_BEGIN
   if (!plan) return fail(PE_ERR_INVALID, "null plan");
   const hipStream_t st = static_cast<hipStream_t>(stream);
   if (plan->fused.nblocks) {
@@ -308,24 +327,30 @@ pe_status pe_end_plan_run(const pe_end_plan* plan, void* stream) {
     if (s != PE_OK) return s;
   }
   return PE_OK;
+  PE_GUARD_END
 }
 
 int64_t pe_end_plan_nodes(const pe_end_plan* plan) { return plan ? plan->nodes : -1; }
 
 pe_status pe_end_plan_destroy(pe_end_plan* plan) {
+  PE_GUARD_BEGIN
   if (!plan) return PE_OK;
   for (auto& p : plan->passes) plan_destroy(&p);
   fused_plan_destroy(&plan->fused);
   delete plan;
   return PE_OK;
+  PE_GUARD_END
 }
 
 pe_status pe_end_compute_line(const pe_line* line, const void* d_input, void* d_output, void* stream) {
+  PE_GUARD_BEGIN
   return pe_end_compute_lines(1, line, &d_input, &d_output, stream);
+  PE_GUARD_END
 }
 
 pe_status pe_end_schedule_line(const pe_line* line, const void* d_input, void* d_output, void* stream,
                                void (*wakeup)(void*), void* ctx) {
+  PE_GUARD_BEGIN
   const pe_status s = pe_end_compute_line(line, d_input, d_output, stream);
   if (s != PE_OK || !wakeup) return s;
   auto* w = new WakeupCtx{wakeup, ctx};
@@ -334,9 +359,11 @@ pe_status pe_end_schedule_line(const pe_line* line, const void* d_input, void* d
     return fail(PE_ERR_DEVICE, "hipLaunchHostFunc failed");
   }
   return PE_OK;
+  PE_GUARD_END
 }
 
 pe_status pe_end_compute_line_host(const pe_line* line, const void* h_input, void* h_output) {
+  PE_GUARD_BEGIN
   if (!line || !h_output) return fail(PE_ERR_INVALID, "null argument");
   LineHost h;
   std::string err;
@@ -359,11 +386,14 @@ pe_status pe_end_compute_line_host(const pe_line* line, const void* h_input, voi
   if (din) (void)hipFree(din);
   (void)hipFree(dout);
   return s;
+  PE_GUARD_END
 }
 
 pe_status pe_fill_seeded(void* d_nodes, int64_t nodes, uint64_t seed, int64_t first_node, void* stream) {
+  PE_GUARD_BEGIN
   if (nodes < 0 || (nodes && !d_nodes)) return fail(PE_ERR_INVALID, "bad fill");
   return launch_fill_seeded(d_nodes, nodes, seed, first_node, static_cast<hipStream_t>(stream));
+  PE_GUARD_END
 }
 
 int64_t pe_midsolve_workspace_size(int min_slice) {
@@ -432,6 +462,7 @@ struct pe_mid_plan {
 extern "C" {
 
 pe_status pe_mid_plan_create(int n, const pe_high_board* boards, const pe_mid_config* config, pe_mid_plan** plan) {
+  PE_GUARD_BEGIN
   if (n < 0 || (n && !boards) || !plan) return fail(PE_ERR_INVALID, "bad plan arguments");
   *plan = nullptr;
   std::vector<MidInfo> infos;
@@ -449,10 +480,12 @@ pe_status pe_mid_plan_create(int n, const pe_high_board* boards, const pe_mid_co
   }
   *plan = p;
   return PE_OK;
+  PE_GUARD_END
 }
 
 pe_status pe_mid_schedule_info(int n, const pe_high_board* boards, const pe_mid_config* config, int64_t* launches,
                                int64_t* workspace_bytes) {
+  PE_GUARD_BEGIN
   if (n < 0 || (n && !boards) || !config || config->workspace_budget <= 0 || config->small_spots <= 0 ||
       config->workgroups <= 0 || config->threads <= 0)
     return fail(PE_ERR_INVALID, "schedule info needs a board list and a complete configuration");
@@ -473,23 +506,29 @@ pe_status pe_mid_schedule_info(int n, const pe_high_board* boards, const pe_mid_
   if (launches) *launches = l;
   if (workspace_bytes) *workspace_bytes = int64_t(L.bytes);
   return PE_OK;
+  PE_GUARD_END
 }
 
 pe_status pe_mid_plan_run(pe_mid_plan* plan, void* stream) {
+  PE_GUARD_BEGIN
   if (!plan) return fail(PE_ERR_INVALID, "null plan");
   return mid_plan_run(plan->plan, static_cast<hipStream_t>(stream));
+  PE_GUARD_END
 }
 
 pe_status pe_mid_plan_results(pe_mid_plan* plan, pe_halfsupers (*results)[19], void* stream) {
+  PE_GUARD_BEGIN
   if (!plan || (plan->plan.nb && !results)) return fail(PE_ERR_INVALID, "null argument");
   const hipStream_t s = static_cast<hipStream_t>(stream);
   pe_status st = mid_plan_copy_results(plan->plan, results, s);
   PE_HIP(hipStreamSynchronize(s), "mid plan results");
   if (st == PE_OK) mid_plan_zero_tail(plan->plan, results);
   return st;
+  PE_GUARD_END
 }
 
 pe_status pe_mid_plan_info(const pe_mid_plan* plan, pe_mid_plan_stats* info, pe_mid_config* config) {
+  PE_GUARD_BEGIN
   if (!plan) return fail(PE_ERR_INVALID, "null plan");
   const MidPlan& P = plan->plan;
   if (info) {
@@ -511,16 +550,20 @@ pe_status pe_mid_plan_info(const pe_mid_plan* plan, pe_mid_plan_stats* info, pe_
     config->use_graph = P.cfg.graph;
   }
   return PE_OK;
+  PE_GUARD_END
 }
 
 pe_status pe_mid_plan_destroy(pe_mid_plan* plan) {
+  PE_GUARD_BEGIN
   if (!plan) return PE_OK;
   mid_plan_destroy(&plan->plan);
   delete plan;
   return PE_OK;
+  PE_GUARD_END
 }
 
 pe_status pe_mid_autotune(int n, const pe_high_board* boards, int64_t workspace_budget, pe_mid_config* best) {
+  PE_GUARD_BEGIN
   if (n <= 0 || !boards || !best) return fail(PE_ERR_INVALID, "bad autotune arguments");
   std::vector<MidInfo> infos;
   pe_status st = mid_infos(n, boards, &infos);
@@ -578,45 +621,59 @@ pe_status pe_mid_autotune(int n, const pe_high_board* boards, int64_t workspace_
   g_tuned[dev].workgroups = chosen.workgroups;
   *best = chosen;
   return PE_OK;
+  PE_GUARD_END
 }
 
 pe_status pe_mid_server_create(const pe_mid_server_config* config, pe_mid_server** server) {
+  PE_GUARD_BEGIN
   if (!server) return fail(PE_ERR_INVALID, "null argument");
   MidServer* S = nullptr;
   const pe_status st = mid_server_create(config, &S);
   *server = reinterpret_cast<pe_mid_server*>(S);
   return st;
+  PE_GUARD_END
 }
 
 pe_status pe_mid_server_submit(pe_mid_server* server, const pe_high_board* board, int64_t* ticket) {
+  PE_GUARD_BEGIN
   if (!server || !board || !ticket) return fail(PE_ERR_INVALID, "null argument");
   return mid_server_submit(reinterpret_cast<MidServer*>(server), *board, ticket);
+  PE_GUARD_END
 }
 
 pe_status pe_mid_server_poll(pe_mid_server* server, int64_t ticket, pe_halfsupers results[19], int* done,
                              double* latency_ms) {
+  PE_GUARD_BEGIN
   if (!server || !done) return fail(PE_ERR_INVALID, "null argument");
   return mid_server_poll(reinterpret_cast<MidServer*>(server), ticket, results, done, latency_ms);
+  PE_GUARD_END
 }
 
 pe_status pe_mid_server_wait(pe_mid_server* server, int64_t ticket, pe_halfsupers results[19], double* latency_ms) {
+  PE_GUARD_BEGIN
   if (!server) return fail(PE_ERR_INVALID, "null argument");
   return mid_server_wait(reinterpret_cast<MidServer*>(server), ticket, results, latency_ms);
+  PE_GUARD_END
 }
 
 pe_status pe_mid_server_stats(pe_mid_server* server, pe_mid_server_info* stats) {
+  PE_GUARD_BEGIN
   if (!server || !stats) return fail(PE_ERR_INVALID, "null argument");
   mid_server_stats(reinterpret_cast<MidServer*>(server), stats);
   return PE_OK;
+  PE_GUARD_END
 }
 
 pe_status pe_mid_server_destroy(pe_mid_server* server) {
+  PE_GUARD_BEGIN
   if (server) mid_server_destroy(reinterpret_cast<MidServer*>(server));
   return PE_OK;
+  PE_GUARD_END
 }
 
 pe_status pe_midsolve_batch(int n, const pe_high_board* boards, pe_halfsupers (*results)[19], int mode,
                             void* stream) {
+  PE_GUARD_BEGIN
   if (n < 0 || (n && (!boards || !results))) return fail(PE_ERR_INVALID, "bad batch");
   if (mode != 0 && mode != 1) return fail(PE_ERR_INVALID, "unsupported batch mode");
   if (mode == 1) {
@@ -667,36 +724,45 @@ pe_status pe_midsolve_batch(int n, const pe_high_board* boards, pe_halfsupers (*
   }
   PE_HIP(hipStreamSynchronize(s), "midsolve batch");
   return st;
+  PE_GUARD_END
 }
 
 pe_status pe_midsolve_internal(const pe_high_board* board, pe_halfsupers results[19]) {
+  PE_GUARD_BEGIN
   if (!board || !results) return fail(PE_ERR_INVALID, "null argument");
   pe_halfsupers (*r)[19] = reinterpret_cast<pe_halfsupers (*)[19]>(results);
   return pe_midsolve_batch(1, board, r, 0, nullptr);
+  PE_GUARD_END
 }
 
 pe_status pe_high_done(const pe_high_board* board, int* done, int* value) {
+  PE_GUARD_BEGIN
   if (!board || !done || !value) return fail(PE_ERR_INVALID, "null argument");
   if ((board->side[0] & board->side[1]) || board->ply > 72) return fail(PE_ERR_INVALID, "invalid board");
   *done = high_done_and_value(*board, value) ? 1 : 0;
   return PE_OK;
+  PE_GUARD_END
 }
 
 pe_status pe_midsolve_traverse(const pe_high_board* board, const pe_halfsupers* results, pe_mid_value* values,
                                int* count) {
+  PE_GUARD_BEGIN
   if (!board || !results || !values || !count) return fail(PE_ERR_INVALID, "null argument");
   MidInfo I;
   if (!make_mid_info(*board, &I)) return fail(PE_ERR_INVALID, "invalid board");
   *count = mid_traverse(*board, results, values);
   return PE_OK;
+  PE_GUARD_END
 }
 
 pe_status pe_midsolve(const pe_high_board* board, pe_mid_value* values, int* count) {
+  PE_GUARD_BEGIN
   if (!board || !values || !count) return fail(PE_ERR_INVALID, "null argument");
   pe_halfsupers results[19];
   const pe_status s = pe_midsolve_internal(board, results);
   if (s != PE_OK) return s;
   return pe_midsolve_traverse(board, results, values, count);
+  PE_GUARD_END
 }
 
 }  // extern "C"

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <new>
+#include <stdexcept>
 #include <string>
 
 #include "../../include/pentago_amd.h"
@@ -41,6 +43,19 @@ struct DeviceBuffer {
     return e;
   }
 };
+// Every C entry point's body runs inside PE_GUARD_BEGIN / PE_GUARD_END: no C++ exception
+// (a failed host allocation, above all) crosses the C ABI; it becomes an error status.
+#define PE_GUARD_BEGIN try {
+#define PE_GUARD_END                                                                \
+  }                                                                                 \
+  catch (const std::bad_alloc&) {                                                   \
+    ::pe::set_error("host memory exhausted");                                       \
+    return PE_ERR_CAPACITY;                                                         \
+  }                                                                                 \
+  catch (const std::exception& e_) {                                                \
+    ::pe::set_error(e_.what());                                                     \
+    return PE_ERR_INVALID;                                                          \
+  }
 #define PE_HIP(expr, msg)                                                           \
   do {                                                                              \
     hipError_t e_ = (expr);                                                         \

@@ -126,13 +126,16 @@ struct pe_slice {
 extern "C" {
 
 pe_status pe_end_count_blocks(const pe_slab* slab, uint64_t* d_counts, void* stream) {
+  PE_GUARD_BEGIN
   if (!slab) return failed(PE_ERR_INVALID, "null slab");
   std::string err;
   const pe_status s = count_blocks_launch(1, slab, &d_counts, nullptr, static_cast<hipStream_t>(stream), &err);
   return s == PE_OK ? s : failed(s, err);
+  PE_GUARD_END
 }
 
 pe_status pe_end_count_blocks_many(int n, const pe_slab* slabs, uint64_t* d_counts, void* d_workspace, void* stream) {
+  PE_GUARD_BEGIN
   if (n < 0 || (n && (!slabs || !d_counts || !d_workspace))) return failed(PE_ERR_INVALID, "bad count arguments");
   std::vector<uint64_t*> dst(n);
   int64_t first = 0;
@@ -143,30 +146,38 @@ pe_status pe_end_count_blocks_many(int n, const pe_slab* slabs, uint64_t* d_coun
   std::string err;
   const pe_status s = count_blocks_launch(n, slabs, dst.data(), d_workspace, static_cast<hipStream_t>(stream), &err);
   return s == PE_OK ? s : failed(s, err);
+  PE_GUARD_END
 }
 
 int64_t pe_end_count_workspace_bytes(int n) { return int64_t(count_desc_bytes(n)); }
 
 pe_status pe_end_count_slabs(int n, const pe_slab* slabs, uint64_t (*counts)[4]) {
+  PE_GUARD_BEGIN
   if (n < 0 || (n && (!slabs || !counts))) return failed(PE_ERR_INVALID, "bad count arguments");
   return count_slabs(n, slabs, counts);
+  PE_GUARD_END
 }
 
 pe_status pe_supertensor_decode(int filter, int turn, const void* d_in, int64_t nodes, void* d_out, void* stream) {
+  PE_GUARD_BEGIN
   std::string err;
   const pe_status s = launch_supertensor_format(true, filter, turn, d_in, nodes, d_out,
                                                 static_cast<hipStream_t>(stream), &err);
   return s == PE_OK ? s : failed(s, err);
+  PE_GUARD_END
 }
 
 pe_status pe_supertensor_encode(int filter, int turn, const void* d_in, int64_t nodes, void* d_out, void* stream) {
+  PE_GUARD_BEGIN
   std::string err;
   const pe_status s = launch_supertensor_format(false, filter, turn, d_in, nodes, d_out,
                                                 static_cast<hipStream_t>(stream), &err);
   return s == PE_OK ? s : failed(s, err);
+  PE_GUARD_END
 }
 
 pe_status pe_end_slab_samples(const pe_slab* slab, int n, const uint64_t* boards, void* h_nodes) {
+  PE_GUARD_BEGIN
   if (!slab || n < 0 || (n && (!boards || !h_nodes))) return failed(PE_ERR_INVALID, "bad sample arguments");
   Section s;
   int shape[4];
@@ -199,9 +210,11 @@ pe_status pe_end_slab_samples(const pe_slab* slab, int n, const uint64_t* boards
       return failed(PE_ERR_DEVICE, "sample copy failed");
   }
   return PE_OK;
+  PE_GUARD_END
 }
 
 pe_status pe_descendent_sections(const pe_section* root, int slice, int cap, pe_section* out, int* count) {
+  PE_GUARD_BEGIN
   if (!root || !count || cap < 0 || (cap && !out)) return failed(PE_ERR_INVALID, "bad section arguments");
   Section r;
   int shape[4];
@@ -227,6 +240,7 @@ pe_status pe_descendent_sections(const pe_section* root, int slice, int cap, pe_
   *count = int(found.size());
   for (int k = 0; k < std::min(cap, int(found.size())); k++) std::memcpy(out[k].counts, found[k].c, 8);
   return PE_OK;
+  PE_GUARD_END
 }
 
 int64_t pe_slice_bytes(int n, const pe_section* sections) {
@@ -244,6 +258,7 @@ int64_t pe_slice_bytes(int n, const pe_section* sections) {
 }
 
 pe_status pe_slice_create_in(int n, const pe_section* sections, void* arena, int64_t arena_bytes, pe_slice** out) {
+  PE_GUARD_BEGIN
   if (n <= 0 || !sections || !out || (arena && arena_bytes < 0)) return failed(PE_ERR_INVALID, "bad slice arguments");
   auto* st = new pe_slice();
   (void)hipGetDevice(&st->device);
@@ -283,13 +298,17 @@ pe_status pe_slice_create_in(int n, const pe_section* sections, void* arena, int
   }
   *out = st;
   return PE_OK;
+  PE_GUARD_END
 }
 
 pe_status pe_slice_create(int n, const pe_section* sections, pe_slice** out) {
+  PE_GUARD_BEGIN
   return pe_slice_create_in(n, sections, nullptr, 0, out);
+  PE_GUARD_END
 }
 
 pe_status pe_slice_compute(pe_slice* out, const pe_slice* children, void* stream) {
+  PE_GUARD_BEGIN
   if (!out) return failed(PE_ERR_INVALID, "null slice");
   if (out->slice < 35 && !children) return failed(PE_ERR_INVALID, "slices below 35 need the next slice's store");
   if (children && children->slice != out->slice + 1) return failed(PE_ERR_INVALID, "children are not the next slice");
@@ -303,23 +322,29 @@ pe_status pe_slice_compute(pe_slice* out, const pe_slice* children, void* stream
     s = failed(PE_ERR_DEVICE, "slice compute failed");
   pe_end_plan_destroy(plan);
   return s;
+  PE_GUARD_END
 }
 
 pe_status pe_slice_slab(const pe_slice* s, int k, pe_slab* slab) {
+  PE_GUARD_BEGIN
   if (!s || !slab || k < 0 || k >= int(s->slabs.size())) return failed(PE_ERR_INVALID, "bad slab index");
   *slab = s->slabs[k];
   return PE_OK;
+  PE_GUARD_END
 }
 
 int pe_slice_sections(const pe_slice* s) { return s ? int(s->slabs.size()) : -1; }
 
 pe_status pe_slice_counts(const pe_slice* s, uint64_t (*counts)[4]) {
+  PE_GUARD_BEGIN
   if (!s || !counts) return failed(PE_ERR_INVALID, "null argument");
   return count_slabs(int(s->slabs.size()), s->slabs.data(), counts);
+  PE_GUARD_END
 }
 
 pe_status pe_slice_plan_groups(int n, const pe_section* sections, int64_t budget, int regenerate, int* order,
                                int* ends, int* ngroups) {
+  PE_GUARD_BEGIN
   if (n <= 0 || !sections || !order || !ends || !ngroups || budget <= 0)
     return failed(PE_ERR_INVALID, "bad group arguments");
   std::vector<int64_t> own(n);
@@ -411,6 +436,7 @@ pe_status pe_slice_plan_groups(int n, const pe_section* sections, int64_t budget
   ends[count++] = n;
   *ngroups = count;
   return PE_OK;
+  PE_GUARD_END
 }
 
 }  // extern "C"
@@ -435,19 +461,22 @@ int64_t group_arena_bytes(const pe_section* sections, const std::vector<int>& en
 
 extern "C" {
 
-int64_t pe_slice_arena_bytes(int n, const pe_section* sections, int64_t budget, int regenerate) {
+int64_t pe_slice_arena_bytes(int n, const pe_section* sections, int64_t budget, int regenerate) try {
   std::vector<int> order(std::max(n, 1)), ends(std::max(n, 1));
   int ng = 0;
   if (pe_slice_plan_groups(n, sections, budget, regenerate, order.data(), ends.data(), &ng) != PE_OK) return -1;
   std::vector<pe_section> ps(n);
   for (int k = 0; k < n; k++) ps[k] = sections[order[k]];
   return group_arena_bytes(ps.data(), ends, ng, regenerate != 0);
+} catch (const std::exception&) {
+  return -1;
 }
 
 pe_status pe_slice_grouped_counts_in(int n, const pe_section* sections, int64_t budget, const pe_slice* children,
                                      const pe_slice* grandchildren, uint64_t (*counts)[4],
                                      void (*visit)(void* ctx, const pe_slice* group), void* ctx, int* ngroups,
                                      void* arena, int64_t arena_bytes, void* stream) {
+  PE_GUARD_BEGIN
   if (!counts || !arena) return failed(PE_ERR_INVALID, "null counts or arena");
   if (!children && !grandchildren) return failed(PE_ERR_INVALID, "need the next slice's store or the one after it");
   const bool regen = !children;
@@ -530,12 +559,14 @@ pe_status pe_slice_grouped_counts_in(int n, const pe_section* sections, int64_t 
     for (int k = 0; k < n; k++)
       for (int j = 0; j < 4; j++) counts[order[k]][j] = pcounts[k][j];
   return st;
+  PE_GUARD_END
 }
 
 pe_status pe_slice_grouped_counts(int n, const pe_section* sections, int64_t budget, const pe_slice* children,
                                   const pe_slice* grandchildren, uint64_t (*counts)[4],
                                   void (*visit)(void* ctx, const pe_slice* group), void* ctx, int* ngroups,
                                   void* stream) {
+  PE_GUARD_BEGIN
   if (!counts) return failed(PE_ERR_INVALID, "null counts");
   if (!children && !grandchildren) return failed(PE_ERR_INVALID, "need the next slice's store or the one after it");
   const int64_t bytes = pe_slice_arena_bytes(n, sections, budget, !children);
@@ -547,13 +578,16 @@ pe_status pe_slice_grouped_counts(int n, const pe_section* sections, int64_t bud
                                                   ngroups, arena, bytes, stream);
   (void)hipFree(arena);
   return st;
+  PE_GUARD_END
 }
 
 pe_status pe_slice_destroy(pe_slice* s) {
+  PE_GUARD_BEGIN
   if (!s) return PE_OK;
   for (void* d : s->data) (void)hipFree(d);
   delete s;
   return PE_OK;
+  PE_GUARD_END
 }
 
 }  // extern "C"
