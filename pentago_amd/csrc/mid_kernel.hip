@@ -37,10 +37,34 @@ __global__ __launch_bounds__(256) void mid_prep_kernel(const MidBoardDev* __rest
   mid_prep_entry(boards[b], t - prefix[b], T);
 }
 
+// Level launches of one board are spread over the 8 XCDs (workgroup i runs on XCD i % 8) by
+// bands of s1p: XCD x sweeps every s0 for bands x, x + 8, ... in turn.  A position's
+// children sit in the rows of the next level at columns that follow its s1p, so an XCD's L2
+// then holds its band of each row instead of every XCD touching every column.
+__device__ __forceinline__ bool banded_position(uint32_t wg, uint32_t tid, uint32_t band, uint32_t nsets1p,
+                                                uint32_t nsets0, int64_t* pos) {
+  const int64_t q = int64_t(wg >> 3) * 256 + tid;
+  const int64_t per_band = int64_t(nsets0) * band;
+  const int64_t bi = q / per_band;
+  const int64_t r = q - bi * per_band;
+  const int64_t s0 = r / band;
+  const int64_t s1p = (int64_t(wg & 7) + 8 * bi) * band + (r - s0 * band);
+  if (s1p >= nsets1p) return false;   // a band past the last, or the last band's tail
+  *pos = s0 * nsets1p + s1p;
+  return true;
+}
+
 // step: level n = spots_b - step for every board with spots_b >= step
+// band > 0 (one board only): banded order, see banded_position
 __global__ __launch_bounds__(256) void mid_level_kernel(const MidBoardDev* __restrict__ boards,
                                                         const int64_t* __restrict__ prefix, int nb,
-                                                        int64_t total, int step, DevTables T) {
+                                                        int64_t total, int step, LevelGrid g, DevTables T) {
+  if (g.band) {
+    int64_t pos;
+    if (!banded_position(blockIdx.x, threadIdx.x, g.band, g.nsets1p, g.nsets0, &pos)) return;
+    mid_position(boards[0], boards[0].I.spots - step, pos, boards[0].work, T);
+    return;
+  }
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= total) return;
   const int b = find_board(prefix, nb, t);
@@ -49,13 +73,35 @@ __global__ __launch_bounds__(256) void mid_level_kernel(const MidBoardDev* __res
 }
 
 __global__ __launch_bounds__(256) void mid_level1_kernel(const MidBoardDev* __restrict__ board, int n,
-                                                         int64_t positions, DevTables T) {
-  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= positions) return;
+                                                         int64_t positions, LevelGrid g, DevTables T) {
+  int64_t t;
+  if (g.band) {
+    if (!banded_position(blockIdx.x, threadIdx.x, g.band, g.nsets1p, g.nsets0, &t)) return;
+  } else {
+    t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= positions) return;
+  }
   mid_position(*board, n, t, board->work, T);
 }
 
 }  // namespace
+
+LevelGrid level_grid(const MidInfo& I, int n) {
+  const int64_t positions = I.positions[n];
+  const uint32_t n1 = uint32_t(I.sets1p_off[n + 1] - I.sets1p_off[n]);
+  LevelGrid g;
+  g.blocks = unsigned((positions + 255) / 256);
+  g.band = 0;
+  g.nsets1p = n1;
+  g.nsets0 = n1 ? uint32_t(positions / n1) : 0;
+  const uint32_t band = std::min<uint32_t>(kLevelBand, (n1 + 7) / 8);
+  if (kLevelBand == 0 || band < 16 || positions < (int64_t(1) << 16)) return g;   // small levels: linear order
+  const uint32_t nbands = (n1 + band - 1) / band;
+  const int64_t per_xcd = int64_t((nbands + 7) / 8) * g.nsets0 * band;
+  g.band = band;
+  g.blocks = unsigned(8 * ((per_xcd + 255) / 256));
+  return g;
+}
 
 // Allocation of one board's device scratch inside a caller-provided arena.
 static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
@@ -86,11 +132,11 @@ void mid_carve(MidBoardDev* B, uint8_t* base) {
   B->results = reinterpret_cast<pe_halfsupers*>(base + t + mid_work_bytes(B->I));
 }
 
-pe_status mid_launch_level(const MidBoardDev* board, int n, int64_t positions, const DevTables& T,
-                           hipStream_t stream) {
+pe_status mid_launch_level(const MidBoardDev* board, const MidInfo& info, int n, int64_t positions,
+                           const DevTables& T, hipStream_t stream) {
   if (positions <= 0) return PE_OK;
-  hipLaunchKernelGGL(mid_level1_kernel, dim3(unsigned((positions + 255) / 256)), dim3(256), 0, stream, board, n,
-                     positions, T);
+  const LevelGrid g = level_grid(info, n);
+  hipLaunchKernelGGL(mid_level1_kernel, dim3(g.blocks), dim3(256), 0, stream, board, n, positions, g, T);
   PE_HIP(hipGetLastError(), "mid_level1_kernel launch");
   return PE_OK;
 }
@@ -165,8 +211,12 @@ pe_status mid_solve_levels(const std::vector<MidInfo>& infos, uint8_t* arena, st
     const int64_t* p = prefix.data() + size_t(nb + 1) * (t + 1);
     const int64_t total = p[nb];
     if (total <= 0) continue;
-    hipLaunchKernelGGL(mid_level_kernel, dim3(unsigned((total + 255) / 256)), dim3(256), 0, stream, dboards,
-                       dprefix + size_t(nb + 1) * (t + 1), nb, total, t, T);
+    LevelGrid g;
+    g.blocks = unsigned((total + 255) / 256);
+    g.band = g.nsets1p = g.nsets0 = 0;
+    if (nb == 1) g = level_grid(infos[0], infos[0].spots - t);
+    hipLaunchKernelGGL(mid_level_kernel, dim3(g.blocks), dim3(256), 0, stream, dboards,
+                       dprefix + size_t(nb + 1) * (t + 1), nb, total, t, g, T);
     PE_HIP(hipGetLastError(), "mid_level_kernel launch");
   }
   if (!desc_cache) PE_HIP(hipFreeAsync(d, stream), "hipFreeAsync(mid descriptors)");

@@ -239,7 +239,7 @@ pe_status launch_big(const MidPlan& plan, size_t k, hipStream_t s) {
   pe_status st = mid_launch_prep(B, plan.prefix + L.small.size() + 1 + 2 * k, 1, plan.big_prep_total[k], plan.tables, s);
   if (st != PE_OK) return st;
   for (int n = I.spots; n >= 0; n--) {
-    st = mid_launch_level(B, n, I.positions[n], plan.tables, s);
+    st = mid_launch_level(B, I, n, I.positions[n], plan.tables, s);
     if (st != PE_OK) return st;
   }
   return PE_OK;
