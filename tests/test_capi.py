@@ -88,3 +88,14 @@ def test_seeded_generator_statistics():
     bits = lambda a: np.unpackbits(a.view(np.uint8)).mean()
     assert abs(bits(known) - 58982 / 65536) < 0.003
     assert abs(bits(win) - 58982 / 65536 / 2) < 0.003
+
+
+def test_count_many_arguments_checked_on_the_host():
+    # descriptor workspace grows with the slab count; missing buffers are rejected before any
+    # device call
+    assert 0 < P.count_workspace_bytes(1) <= P.count_workspace_bytes(2) < P.count_workspace_bytes(100)
+    slab = P.Slab.of(((3, 4), (4, 4), (4, 4), (4, 3)), [0] * 4, [1] * 4, 0)
+    with pytest.raises(P.PentagoError):
+        P.count_blocks_many([slab, slab], 1 << 20, None)
+    with pytest.raises(P.PentagoError):
+        P.count_blocks_many([slab], None, 1 << 20)
