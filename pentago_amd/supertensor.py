@@ -32,8 +32,12 @@ BLOCK = 8   # slab blocks are 8^4 nodes; files written here use the same block s
 
 
 def decompress(data, size):
-    """Decompress one blob (zlib or xz by magic, pentago/data/compress.cc:71-95)."""
-    out = lzma.decompress(data, format=lzma.FORMAT_XZ) if data[:6] == XZ_MAGIC else zlib.decompress(data)
+    """Decompress one blob (zlib or xz by magic, pentago/data/compress.cc:71-95).  Output is
+    capped one byte past the expected size, so a corrupt blob cannot inflate without bound."""
+    if data[:6] == XZ_MAGIC:
+        out = lzma.LZMADecompressor(format=lzma.FORMAT_XZ).decompress(data, max_length=size + 1)
+    else:
+        out = zlib.decompressobj().decompress(data, size + 1)
     if len(out) != size:
         raise PentagoError("supertensor blob decompressed to %d bytes, expected %d" % (len(out), size))
     return out
