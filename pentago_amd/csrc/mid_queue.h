@@ -108,8 +108,9 @@ pe_status mid_launch_prep(const MidBoardDev* boards, const int64_t* prefix, int 
                           const DevTables& T, hipStream_t stream);
 // Launch shape of one level of one board: `band` > 0 spreads the level over the XCDs by bands
 // of s1p columns (blocks counts the padded grid), 0 keeps the linear order.  Measured on
-// 2048 boards of 18..30 stones, batched: linear 1.14 s; bands of 4 / 8 / 16 / 32 columns
-// 1.22 / 0.99 / 0.95 / 0.99 s, one band per XCD 1.01 s.
+// 2048 boards of 18..30 stones, batched: linear 1.14 s; bands of 4 / 8 / 12 / 16 / 24 / 32
+// columns 1.22 / 0.99 / 0.96 / 0.95 / 0.98 / 0.99 s, one band per XCD 1.01 s; bands 16 wide
+// dealt to the XCDs in contiguous runs instead of in turn 1.05 s.
 constexpr uint32_t kLevelBand = 16;
 struct LevelGrid {
   unsigned blocks;
