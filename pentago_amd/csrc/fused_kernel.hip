@@ -457,6 +457,13 @@ pe_status fused_plan_create(int nout, const pe_slab* outs, int nch, const pe_sla
     for (int k; (k = next.fetch_add(1)) < nout;) build_slab(outs[k], nch, ch, &builds[size_t(k)]);
   };
   std::vector<std::thread> pool;
+  struct Join {   // joins the started threads on every way out, a failed thread start included
+    std::vector<std::thread>& p;
+    ~Join() {
+      for (auto& th : p)
+        if (th.joinable()) th.join();
+    }
+  } join{pool};
   for (int t = 1; t < nthreads; t++) pool.emplace_back(work);
   work();
   for (auto& th : pool) th.join();
