@@ -134,8 +134,8 @@ typedef struct {
  *         once -- the line kernel of pentago/end/compute.cc:219-371 for all four lines
  *         through a block, merged as pentago/end/block_store.cc:173-229);
  * mode 1: four line passes, the later ones OR-accumulating (for comparison).
- * Output sections with 35 stones (one open dimension, no child data) always run as one line
- * pass. */
+ * Output sections with 35 stones (one open dimension, no child data; the move fills the
+ * board, compute.cc:329-332) run on the fused kernel's no-children form in mode 0. */
 pe_status pe_end_slab_plan_create(const pe_slab* output, int nchildren, const pe_slab* children, int mode,
                                   pe_end_plan** plan);
 /* The same for several output slabs (of one or several sections) in one plan: mode 0 runs
@@ -205,8 +205,8 @@ pe_status pe_slice_create_in(int n, const pe_section* sections, void* arena, int
 /* Device bytes the slabs of these sections take (no allocation). */
 int64_t pe_slice_bytes(int n, const pe_section* sections);
 /* Compute every section of `out` from `children` (the store of the next slice holding every
- * child section they read; NULL when `out` is slice 35): one fused plan over all sections
- * (line passes at slice 35), run on `stream` and waited for. */
+ * child section they read; NULL when `out` is slice 35): one fused plan over all sections,
+ * run on `stream` and waited for. */
 pe_status pe_slice_compute(pe_slice* out, const pe_slice* children, void* stream);
 /* Section k's slab (k < sections), valid until pe_slice_destroy. */
 pe_status pe_slice_slab(const pe_slice* s, int k, pe_slab* slab);

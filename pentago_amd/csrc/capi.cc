@@ -208,6 +208,7 @@ pe_status pe_end_compute_lines(int n, const pe_line* lines, const void* const* d
 
 struct pe_end_plan {
   FusedPlan fused;               // fused block plan (run first when present)
+  FusedPlan fused35;             // fused plan of 35-stone output sections (no child data)
   std::vector<EndPlan> passes;   // then line plans, in order
   int64_t nodes = 0;
 };
@@ -279,17 +280,21 @@ pe_status pe_end_slabs_plan_create(int noutputs, const pe_slab* outputs, int nch
   auto* plan = new pe_end_plan();
   std::string err;
   pe_status s = PE_OK;
-  // mode 0: sections with at most 34 stones go to the fused kernel; 35-stone sections (one
-  // open dimension, no child data) always run as line passes
-  std::vector<pe_slab> fused, lined;
+  // mode 0: every section goes to the fused kernel, 35-stone sections (one open dimension,
+  // no child data) in a plan of their own; mode 1: line passes
+  std::vector<pe_slab> fused, fused35, lined;
   for (int k = 0; k < noutputs; k++) {
     int sum = 0;
     for (int q = 0; q < 4; q++) sum += outputs[k].section.counts[q][0] + outputs[k].section.counts[q][1];
-    (mode == 0 && sum <= 34 ? fused : lined).push_back(outputs[k]);
+    (mode == 1 ? lined : sum == 35 ? fused35 : fused).push_back(outputs[k]);
   }
   if (!fused.empty()) {
     s = fused_plan_create(int(fused.size()), fused.data(), nchildren, children, &plan->fused, &err);
     plan->nodes = plan->fused.output_nodes;
+  }
+  if (!fused35.empty() && s == PE_OK) {
+    s = fused_plan_create(int(fused35.size()), fused35.data(), nchildren, children, &plan->fused35, &err);
+    plan->nodes += plan->fused35.output_nodes;
   }
   // line passes: pass p of every such slab in one launch
   std::vector<std::vector<LineHost>> groups;
@@ -306,6 +311,7 @@ pe_status pe_end_slabs_plan_create(int noutputs, const pe_slab* outputs, int nch
   if (s != PE_OK) {
     for (auto& p : plan->passes) plan_destroy(&p);
     fused_plan_destroy(&plan->fused);
+    fused_plan_destroy(&plan->fused35);
     delete plan;
     return fail(s, err.empty() ? pe_last_error() : err);
   }
@@ -318,10 +324,11 @@ pe_status pe_end_plan_run(const pe_end_plan* plan, void* stream) {
   PE_GUARD_BEGIN
   if (!plan) return fail(PE_ERR_INVALID, "null plan");
   const hipStream_t st = static_cast<hipStream_t>(stream);
-  if (plan->fused.nblocks) {
-    const pe_status s = fused_plan_run(plan->fused, st);
-    if (s != PE_OK) return s;
-  }
+  for (const FusedPlan* f : {&plan->fused, &plan->fused35})
+    if (f->nblocks) {
+      const pe_status s = fused_plan_run(*f, st);
+      if (s != PE_OK) return s;
+    }
   for (const auto& p : plan->passes) {
     const pe_status s = plan_run(p, st);
     if (s != PE_OK) return s;
@@ -337,6 +344,7 @@ pe_status pe_end_plan_destroy(pe_end_plan* plan) {
   if (!plan) return PE_OK;
   for (auto& p : plan->passes) plan_destroy(&p);
   fused_plan_destroy(&plan->fused);
+  fused_plan_destroy(&plan->fused35);
   delete plan;
   return PE_OK;
   PE_GUARD_END

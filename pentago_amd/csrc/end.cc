@@ -98,6 +98,7 @@ pe_status section_move_tables(const Section& s, std::vector<uint16_t>* out, int3
     move_off[d] = -1;
     nmoves[d] = 9 - s.c[d][0] - s.c[d][1];
     if (nmoves[d] <= 0) { nmoves[d] = 0; continue; }
+    if (s.sum() == 35) continue;   // the move fills the board: no child data to address
     // the child side of the moves depends only on (section, d): use the line at block 0
     pe_line L;
     std::memset(&L, 0, sizeof L);

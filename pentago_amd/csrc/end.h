@@ -105,6 +105,7 @@ struct BlockDev {
 struct FusedPlan {
   void* blob = nullptr;
   int nblocks = 0;
+  bool s35 = false;   // blocks of 35-stone sections (the kernel's no-children form)
   size_t off_lines = 0, off_secs = 0, off_fields = 0, off_moves = 0, off_refl = 0;
   int64_t output_nodes = 0;
 };

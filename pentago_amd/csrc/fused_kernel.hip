@@ -100,6 +100,8 @@ __device__ __forceinline__ void g3(const NodeFields& F, int k, int a, int fa, in
 constexpr int kThreads = 256;
 constexpr int kParts = 4096 / kThreads;
 
+// S35: blocks of 35-stone sections, whose one move fills the board (no child data).
+template <bool S35>
 __global__ __launch_bounds__(kThreads)
 void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __restrict__ lines,
                          const SecDev* __restrict__ secs, const uint64_t* __restrict__ fields,
@@ -140,7 +142,7 @@ void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __r
   S4 t0 = s4_zero(), t1 = s4_zero();
 #pragma unroll
   for (int d = 0; d < 4; d++) {
-    if (B.line[d] < 0) continue;
+    if (S35 || B.line[d] < 0) continue;
     const LineDev& L = lines[B.line[d]];
     const int cd = L.child_dim;
     int base = 0, base_last = 0, local_sym = 0;
@@ -211,7 +213,7 @@ void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __r
     for (int t = 0; t < 3; t++) Cv[t] |= x[t];
     PE_FENCE();
   }
-  S4 o0, o1;
+  S4 o0, o1, wv, w0v, w1v;
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     uint64_t w[3], x[3], y[3];
@@ -225,10 +227,28 @@ void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __r
     PE_FENCE();
     g3(F, k, 1, FHI, 3, FAS, 2, FHI, x);
     for (int t = 0; t < 3; t++) w[t] |= red0(x[t]);
-    const uint64_t imm = w[0] | w[1];
-    o0.w[k] = ((w[2] | r1.w[k]) & ~imm) | (w[0] & ~w[1]);
-    o1.w[k] = (r0.w[k] & ~imm) | w[0];
+    if (S35) {
+      // OR over the moves of super_wins(side0 + move) is w[2] | w[0] exactly: groups through
+      // the moved quadrant are in w[2], the rest are groups of side0 (a move only adds wins)
+      wv.w[k] = w[2] | w[0];
+      w0v.w[k] = w[0];
+      w1v.w[k] = w[1];
+    } else {
+      const uint64_t imm = w[0] | w[1];
+      o0.w[k] = ((w[2] | r1.w[k]) & ~imm) | (w[0] & ~w[1]);
+      o1.w[k] = (r0.w[k] & ~imm) | w[0];
+    }
     PE_FENCE();
+  }
+  if (S35) {   // compute.cc:329-332: after the board fills, a position nobody has won is a tie
+    const S4 win = s4_or(wv, rmax(s4_andnot(wv, w1v)));
+    const S4 nl = rmax(s4_or(wv, s4_not(w1v)));
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint64_t imm = w0v.w[k] | w1v.w[k];
+      o0.w[k] = (win.w[k] & ~imm) | (w0v.w[k] & ~w1v.w[k]);
+      o1.w[k] = (nl.w[k] & ~imm) | w0v.w[k];
+    }
   }
   uint64_t* o = B.out + 8 * p;
   if (B.accumulate) {   // OR-merge with other dimensions' lines (block_store.cc:173-229)
@@ -331,8 +351,8 @@ static pe_status pack_fused_plan(const std::vector<BlockDev>& blocks, const std:
   for (size_t k = 0; k < lines.size(); k++) {
     devs[k] = lines[k].dev;
     // child blocks along the child dimension sit at affine offsets in their slab
-    const std::vector<int64_t>& io = lines[k].in_off;
-    devs[k].in_base = io[0];
+    const std::vector<int64_t>& io = lines[k].in_off;   // empty for 35-stone lines (no child data)
+    devs[k].in_base = io.empty() ? 0 : io[0];
     devs[k].in_kstride = io.size() > 1 ? io[1] - io[0] : 0;
     for (size_t kk = 0; kk < io.size(); kk++)
       if (io[kk] != devs[k].in_base + int64_t(kk) * devs[k].in_kstride) {
@@ -409,7 +429,7 @@ static void build_slab(const pe_slab& out, int nch, const pe_slab* ch, SlabBuild
   if (r->status != PE_OK) return;
   Section s;
   std::memcpy(s.c, out.section.counts, 8);
-  if (s.sum() > 34) { r->err = "the fused kernel needs sections with at most 34 stones"; r->status = PE_ERR_INVALID; return; }
+  if (s.sum() > 35) { r->err = "the fused kernel needs sections with at most 35 stones"; r->status = PE_ERR_INVALID; return; }
   SecDev& sec = r->sec;
   std::memset(&sec, 0, sizeof sec);
   sec.turn = s.sum() & 1;
@@ -448,6 +468,16 @@ pe_status fused_plan_create(int nout, const pe_slab* outs, int nch, const pe_sla
                             std::string* err) {
   DevTables T;
   if (device_tables(&T, err) != PE_OK) return PE_ERR_DEVICE;
+  // one plan holds either sections with at most 34 stones or 35-stone sections only
+  bool any35 = false, all35 = true;
+  for (int k = 0; k < nout; k++) {
+    int sum = 0;
+    for (int q = 0; q < 4; q++) sum += outs[k].section.counts[q][0] + outs[k].section.counts[q][1];
+    any35 |= sum == 35;
+    all35 &= sum == 35;
+  }
+  if (any35 && !all35) { *err = "a fused plan mixes 35-stone sections with others"; return PE_ERR_INVALID; }
+  plan->s35 = any35;
   // the slabs' shares are built on host threads (a slice has thousands of sections), then
   // concatenated in slab order, so the plan does not depend on the thread count
   std::vector<SlabBuild> builds(size_t(std::max(nout, 0)));
@@ -591,7 +621,8 @@ pe_status fused_plan_run(const FusedPlan& plan, hipStream_t stream) {
   const int per_xcd = (plan.nblocks + 7) / 8;
   const int64_t grid = int64_t(per_xcd) * kParts * 8;
   if (grid > 0x7fffffff) { set_error("fused plan too large"); return PE_ERR_CAPACITY; }
-  hipLaunchKernelGGL(fused_blocks_kernel, dim3(unsigned(grid)), dim3(kThreads), 0, stream,
+  hipLaunchKernelGGL(plan.s35 ? fused_blocks_kernel<true> : fused_blocks_kernel<false>, dim3(unsigned(grid)),
+                     dim3(kThreads), 0, stream,
                      reinterpret_cast<const BlockDev*>(base), reinterpret_cast<const LineDev*>(base + plan.off_lines),
                      reinterpret_cast<const SecDev*>(base + plan.off_secs),
                      reinterpret_cast<const uint64_t*>(base + plan.off_fields),

@@ -4,7 +4,7 @@
 //
 // A store holds whole sections of one slice on the device, one slab per section covering
 // every block (a 4096-node slot per block).  Computing a slice is one slab plan over all of
-// its sections (the fused kernel; line passes at slice 35) reading the next slice's store.
+// its sections (the fused kernel) reading the next slice's store.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
