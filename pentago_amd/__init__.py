@@ -512,6 +512,18 @@ def midsolve(board):
     return {int(vals[i].raw): int(vals[i].value) for i in range(n.value)}
 
 
+def midsolve_traverse(board, results):
+    """midsolve_traverse (pentago/mid/midengine.cc:127-158) over precomputed results (a
+    (19, 4) uint64 row as MidPlan.results() returns): {raw board: value}."""
+    import numpy as np
+    row = np.ascontiguousarray(results, dtype=np.uint64).reshape(19, 4)
+    vals = (MidValue * 163)()
+    n = C.c_int()
+    _check(lib().pe_midsolve_traverse(C.byref(board), row.ctypes.data_as(C.POINTER(HalfSupers)), vals, C.byref(n)),
+           "pe_midsolve_traverse")
+    return {int(vals[i].raw): int(vals[i].value) for i in range(n.value)}
+
+
 def midsolve_batch(boards, mode=0, stream=None):
     """pe_midsolve_batch: mode 0 level-synchronous launches, 1 persistent work queue."""
     n = len(boards)

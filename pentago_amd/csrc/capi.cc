@@ -67,18 +67,23 @@ pe_status device_tables(DevTables* out, std::string* err) {
     const int r0 = T->rot[s0][0], r1 = T->rot[s1][0];
     stab[i] = (r0 == s0 && r1 == s1) ? 2 : (T->rot[r0][0] == s0 && T->rot[r1][0] == s1) ? 1 : 0;
   }
+  // field pool of the fused kernels (see end.h)
+  std::vector<uint64_t> pool;
+  build_field_pool(&pool);
   const size_t sizes[] = {sizeof T->superwin, sizeof(uint16_t) * size_t(T->rmin_flat_size),
                           sizeof(uint32_t) * sides.size(), sizeof T->rmin_inverse, sizeof T->pack,
-                          sizeof T->commute, sizeof(int32_t) * choose.size(), stab.size()};
+                          sizeof T->commute, sizeof(int32_t) * choose.size(), stab.size(),
+                          sizeof(uint64_t) * pool.size()};
   const void* srcs[] = {T->superwin, T->rmin_flat, sides.data(), T->rmin_inverse, T->pack, T->commute, choose.data(),
-                        stab.data()};
-  constexpr int kParts = 8;
+                        stab.data(), pool.data()};
+  constexpr int kParts = 9;
   size_t total = 0, offs[kParts];
   for (int i = 0; i < kParts; i++) { offs[i] = total; total += (sizes[i] + 255) & ~size_t(255); }
   void* d = nullptr;
   if (hipMalloc(&d, total) != hipSuccess) { *err = "hipMalloc(tables) failed"; return PE_ERR_DEVICE; }
   for (int i = 0; i < kParts; i++)
     if (hipMemcpy(static_cast<uint8_t*>(d) + offs[i], srcs[i], sizes[i], hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipFree(d);
       *err = "hipMemcpy(tables) failed";
       return PE_ERR_DEVICE;
     }
@@ -92,6 +97,7 @@ pe_status device_tables(DevTables* out, std::string* err) {
   D.commute = reinterpret_cast<const uint8_t*>(b + offs[5]);
   D.choose = reinterpret_cast<const int32_t*>(b + offs[6]);
   D.rmin_stab = b + offs[7];
+  D.fields = reinterpret_cast<const uint64_t*>(b + offs[8]);
   g_dev[dev] = D;
   g_dev_ready[dev] = true;
   *out = D;
@@ -241,21 +247,34 @@ pe_status pe_end_plan_create(int n, const pe_line_job* jobs, pe_end_plan** out) 
   }
   // jobs covering whole lines run on the fused kernel (one dimension open per block);
   // partial index ranges, slice 35 and irregular child offsets on the line kernel
-  std::vector<LineHost> fused, lined;
-  for (auto& h : hosts) (fused_line_eligible(h) ? fused : lined).push_back(std::move(h));
+  // jobs covering whole lines run on the fused kernel (one dimension open per block; 35-stone
+  // lines on its no-children form); partial index ranges and irregular child offsets on the
+  // line kernel
+  std::vector<LineHost> fused, fused35, lined;
+  for (auto& h : hosts) (!fused_line_eligible(h) ? lined : h.dev.slice35 ? fused35 : fused).push_back(std::move(h));
   auto* plan = new pe_end_plan();
   std::string err;
   pe_status s = PE_OK;
   if (!fused.empty()) {
     s = fused_line_plan_create(fused, &plan->fused, &err);
-    if (s != PE_OK) { delete plan; return fail(s, err); }
     plan->nodes += plan->fused.output_nodes;
   }
-  if (!lined.empty()) {
+  if (s == PE_OK && !fused35.empty()) {
+    s = fused_line_plan_create(fused35, &plan->fused35, &err);
+    plan->nodes += plan->fused35.output_nodes;
+  }
+  if (s == PE_OK && !lined.empty()) {
     plan->passes.resize(1);
     s = plan_create(lined, &plan->passes[0]);
-    if (s != PE_OK) { fused_plan_destroy(&plan->fused); delete plan; return s; }
-    plan->nodes += plan->passes[0].output_nodes;
+    if (s != PE_OK) err = pe_last_error();
+    else plan->nodes += plan->passes[0].output_nodes;
+  }
+  if (s != PE_OK) {
+    for (auto& p : plan->passes) plan_destroy(&p);
+    fused_plan_destroy(&plan->fused);
+    fused_plan_destroy(&plan->fused35);
+    delete plan;
+    return fail(s, err);
   }
   *out = plan;
   return PE_OK;
@@ -352,17 +371,25 @@ pe_status pe_end_plan_destroy(pe_end_plan* plan) {
 
 pe_status pe_end_compute_line(const pe_line* line, const void* d_input, void* d_output, void* stream) {
   PE_GUARD_BEGIN
-  return pe_end_compute_lines(1, line, &d_input, &d_output, stream);
+  return pe_end_schedule_line(line, d_input, d_output, stream, nullptr, nullptr);
   PE_GUARD_END
 }
 
 pe_status pe_end_schedule_line(const pe_line* line, const void* d_input, void* d_output, void* stream,
                                void (*wakeup)(void*), void* ctx) {
   PE_GUARD_BEGIN
-  const pe_status s = pe_end_compute_line(line, d_input, d_output, stream);
-  if (s != PE_OK || !wakeup) return s;
+  if (!line || !d_output) return fail(PE_ERR_INVALID, "null argument");
+  LineHost h;
+  std::string err;
+  pe_status s = make_line(*line, &h, &err);
+  if (s != PE_OK) return fail(s, err);
+  if (!h.dev.slice35 && !d_input) return fail(PE_ERR_INVALID, "null input for a line with child data");
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  s = fused_line_launch(h, d_input, d_output, 0, st, &err);
+  if (s != PE_OK) return fail(s, err);
+  if (!wakeup) return PE_OK;
   auto* w = new WakeupCtx{wakeup, ctx};
-  if (hipLaunchHostFunc(static_cast<hipStream_t>(stream), wakeup_trampoline, w) != hipSuccess) {
+  if (hipLaunchHostFunc(st, wakeup_trampoline, w) != hipSuccess) {
     delete w;
     return fail(PE_ERR_DEVICE, "hipLaunchHostFunc failed");
   }
@@ -380,20 +407,21 @@ pe_status pe_end_compute_line_host(const pe_line* line, const void* h_input, voi
   const size_t in_bytes = size_t(h.info.input_nodes) * 64, out_bytes = size_t(h.info.output_nodes) * 64;
   if (in_bytes && !h_input) return fail(PE_ERR_INVALID, "null input");
   void *din = nullptr, *dout = nullptr;
+  struct Free {   // the device buffers are released on every way out
+    void*& a; void*& b;
+    ~Free() { if (a) (void)hipFree(a); if (b) (void)hipFree(b); }
+  } free_bufs{din, dout};
   PE_HIP(hipMalloc(&dout, out_bytes), "hipMalloc(output)");
   if (in_bytes) {
-    if (hipMalloc(&din, in_bytes) != hipSuccess) { (void)hipFree(dout); return fail(PE_ERR_DEVICE, "hipMalloc(input)"); }
+    PE_HIP(hipMalloc(&din, in_bytes), "hipMalloc(input)");
     PE_HIP(hipMemcpy(din, h_input, in_bytes, hipMemcpyHostToDevice), "hipMemcpy(input)");
   }
   PE_HIP(hipMemset(dout, 0, out_bytes), "hipMemset(output)");
   s = pe_end_compute_line(line, din, dout, nullptr);
-  if (s == PE_OK) {
-    PE_HIP(hipStreamSynchronize(nullptr), "line compute");
-    PE_HIP(hipMemcpy(h_output, dout, out_bytes, hipMemcpyDeviceToHost), "hipMemcpy(output)");
-  }
-  if (din) (void)hipFree(din);
-  (void)hipFree(dout);
-  return s;
+  if (s != PE_OK) return s;
+  PE_HIP(hipStreamSynchronize(nullptr), "line compute");
+  PE_HIP(hipMemcpy(h_output, dout, out_bytes, hipMemcpyDeviceToHost), "hipMemcpy(output)");
+  return PE_OK;
   PE_GUARD_END
 }
 

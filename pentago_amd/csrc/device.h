@@ -21,6 +21,7 @@ struct DevTables {
   const uint8_t* commute;       // 8*256 entries
   const int32_t* choose;        // choose(n, k) at [n*32 + k], n,k < 32
   const uint8_t* rmin_stab;     // per rotation-minimal quadrant: log2 of its local stabilizer size
+  const uint64_t* fields;       // field pool of the fused end kernels (end.h: field_pool_off)
 };
 
 // Upload the tables to the current device (idempotent per device).

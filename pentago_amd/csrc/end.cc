@@ -59,36 +59,45 @@ inline int invert_global(int g) { return g ^ (((~g >> 2) & (g & 1)) << 1); }
 
 }  // namespace
 
-void section_field_tables(const Section& s, std::vector<uint64_t>* out, int32_t field_off[4]) {
+int64_t field_pool_words() { return int64_t(16) * tables().rmin_flat_size * 60; }
+
+int32_t field_pool_off(int q, int turn, int moves, int black, int white) {
   const auto& T = tables();
-  const int turn = s.sum() & 1;
-  out->clear();
-  for (int q = 0; q < 4; q++) {
-    field_off[q] = int32_t(out->size() / 4);
-    const int b = rmin_bucket(s.c[q][0], s.c[q][1]);
-    for (int i = T.rmin_offsets[b]; i < T.rmin_offsets[b + 1]; i++) {
-      const uint16_t quad = T.rmin_flat[i];
-      const uint16_t mover = T.unpack[quad][turn], other = T.unpack[quad][1 - turn];
-      uint64_t fm[5][4] = {};
-      for (int e = 0; e < 9; e++) {
-        if ((mover | other) >> e & 1) continue;
-        const uint64_t* f = T.superwin + size_t((q * 512 + (mover | 1 << e)) * 5) * 4;
-        for (int k = 0; k < 20; k++) fm[k / 4][k % 4] |= f[k];
+  const int b = rmin_bucket(black, white);
+  return int32_t(((q * 2 + turn) * 2 + (moves ? 1 : 0)) * T.rmin_flat_size * 15 + T.rmin_offsets[b] * 15);
+}
+
+void build_field_pool(std::vector<uint64_t>* out) {
+  const auto& T = tables();
+  out->assign(size_t(field_pool_words()), 0);
+  for (int q = 0; q < 4; q++)
+    for (int turn = 0; turn < 2; turn++)
+      for (int moves = 0; moves < 2; moves++) {
+        uint64_t* o = out->data() + size_t(((q * 2 + turn) * 2 + moves) * T.rmin_flat_size) * 60;
+        for (int i = 0; i < T.rmin_flat_size; i++, o += 60) {
+          const uint16_t quad = T.rmin_flat[i];
+          const uint16_t mover = T.unpack[quad][turn], other = T.unpack[quad][1 - turn];
+          uint64_t fm[5][4] = {};
+          for (int e = 0; e < 9 && moves; e++) {
+            if ((mover | other) >> e & 1) continue;
+            const uint64_t* f = T.superwin + size_t((q * 512 + (mover | 1 << e)) * 5) * 4;
+            for (int k = 0; k < 20; k++) fm[k / 4][k % 4] |= f[k];
+          }
+          const uint64_t* f0 = T.superwin + size_t((q * 512 + mover) * 5) * 4;
+          const uint64_t* f1 = T.superwin + size_t((q * 512 + other) * 5) * 4;
+          // word-major: for word k, table t (F0, F1, Fm), field f
+          for (int k = 0; k < 4; k++)
+            for (int f = 0; f < 5; f++) {
+              o[15 * k + f] = f0[4 * f + k];
+              o[15 * k + 5 + f] = f1[4 * f + k];
+              o[15 * k + 10 + f] = fm[f][k];
+            }
+        }
       }
-      const uint64_t* f0 = T.superwin + size_t((q * 512 + mover) * 5) * 4;
-      const uint64_t* f1 = T.superwin + size_t((q * 512 + other) * 5) * 4;
-      // word-major: for word k, table t (F0, F1, Fm), field f
-      for (int k = 0; k < 4; k++) {
-        for (int f = 0; f < 5; f++) out->push_back(f0[4 * f + k]);
-        for (int f = 0; f < 5; f++) out->push_back(f1[4 * f + k]);
-        for (int f = 0; f < 5; f++) out->push_back(fm[f][k]);
-      }
-    }
-  }
 }
 
 pe_status section_move_tables(const Section& s, std::vector<uint16_t>* out, int32_t move_off[4], int32_t nmoves[4],
-                              std::string* err) {
+                              std::string* err, int only) {
   const auto& T = tables();
   const int turn = s.sum() & 1;
   int shape[4];
@@ -99,6 +108,7 @@ pe_status section_move_tables(const Section& s, std::vector<uint16_t>* out, int3
     nmoves[d] = 9 - s.c[d][0] - s.c[d][1];
     if (nmoves[d] <= 0) { nmoves[d] = 0; continue; }
     if (s.sum() == 35) continue;   // the move fills the board: no child data to address
+    if (only >= 0 && d != only) continue;
     // the child side of the moves depends only on (section, d): use the line at block 0
     pe_line L;
     std::memset(&L, 0, sizeof L);

@@ -7,6 +7,8 @@
 // (pentago/end/compute.cc:84-94,134-139).  Its output is `length` blocks of block_stride
 // nodes (compute.cc:147-152).  A node is two supers (we-win, we-win-or-tie), 64 bytes.
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -89,7 +91,7 @@ pe_status make_line(const pe_line& line, LineHost* out, std::string* err);
 struct SecDev {
   int32_t turn;
   int32_t rmin_off[4], rmin_size[4];
-  int32_t field_off[4];     // start (in supers) of quadrant q's field table: 15 supers per rmin index
+  int32_t field_off[4];     // start (in supers) of quadrant q's entries in the field pool: 15 supers each
   int32_t move_off[4];      // start (in u16 entries) of dimension d's move table rows
   int32_t nmoves[4];        // moves per node along dimension d (0: closed)
 };
@@ -106,7 +108,7 @@ struct FusedPlan {
   void* blob = nullptr;
   int nblocks = 0;
   bool s35 = false;   // blocks of 35-stone sections (the kernel's no-children form)
-  size_t off_lines = 0, off_secs = 0, off_fields = 0, off_moves = 0, off_refl = 0;
+  size_t off_lines = 0, off_secs = 0, off_moves = 0, off_refl = 0;
   int64_t output_nodes = 0;
 };
 
@@ -117,10 +119,21 @@ struct FusedPlan {
 bool fused_line_eligible(const LineHost& h);
 pe_status fused_line_plan_create(const std::vector<LineHost>& jobs, FusedPlan* plan, std::string* err);
 
-// Field tables of one section (see SecDev): for quadrant q and rotation-minimal index i,
-// 60 words: for each super word k, tables [F0 (mover's side), F1 (other side), Fm (OR over
-// the mover's moves)] x 5 fields (word k of each super).
-void section_field_tables(const Section& s, std::vector<uint64_t>* out, int32_t field_off[4]);
+// One whole block line on the fused kernel, asynchronous on `stream`, its descriptor passed
+// by value (no device memory per call): input block k at node k * (block_stride + 1),
+// output block k at node k * block_stride (pentago/end/compute.cc:134-152).  Thread-safe.
+pe_status fused_line_launch(const LineHost& h, const void* input, void* output, int accumulate, hipStream_t stream,
+                            std::string* err);
+
+// Field pool (one per device, built once): for quadrant q, player to move `turn` and
+// moves flag m, one entry of 60 words per rotation-minimal quadrant (all buckets, in
+// rmin_flat order): for each super word k, tables [F0 (mover's side), F1 (other side), Fm
+// (OR over the mover's moves in q; zero when m = 0)] x 5 fields (word k of each
+// superwin_info field, pentago/base/superscore.cc:38-96).  A section's field_off[q] is
+// field_pool_off(q, turn, m, counts[q]) in supers (4 words).
+int64_t field_pool_words();
+int32_t field_pool_off(int q, int turn, int moves, int black, int white);
+void build_field_pool(std::vector<uint64_t>* out);
 
 // Move tables of one section (fused kernel): for each open dimension d (move_off[d] >= 0,
 // nmoves[d] = empty cells of quadrant d) and rotation-minimal index i of quadrant d, a row
@@ -128,7 +141,8 @@ void section_field_tables(const Section& s, std::vector<uint64_t>* out, int32_t 
 // dimension (bits 0-5), node index inside that block along it after the reflection swap
 // (bits 6-8) and the move's local rotation restricted to the child dimension (bits 9-10).
 // Everything here depends only on (section, d, i, move), not on the line's other blocks.
+// only >= 0: rows of that dimension only (the others keep move_off -1).
 pe_status section_move_tables(const Section& s, std::vector<uint16_t>* out, int32_t move_off[4], int32_t nmoves[4],
-                              std::string* err);
+                              std::string* err, int only = -1);
 
 }  // namespace pe

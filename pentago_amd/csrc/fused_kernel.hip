@@ -15,7 +15,7 @@
 //     wins0, which ~imm removes anyway, so OR_m super_wins(side0 + m) & ~imm equals
 //     OR_G reduce_G(OR_{d in G} Fm_d,G & AND_{q in G, q != d} F0_q,G) & ~imm, with
 //     Fm = OR over the moves of the moved quadrant's field (AND distributes over OR).
-//     Per-section tables hold F0, F1 and Fm per (quadrant, rotation-minimal index).
+//     The device's field pool holds F0, F1 and Fm per (quadrant, rotation-minimal index).
 //   * rmax(A) | rmax(B) = rmax(A | B) and T_m = R_g o shift_{y_m}: the children are shifted
 //     and ORed per dimension, rearranged once per dimension, and rmax runs once per node.
 #include <hip/hip_runtime.h>
@@ -24,6 +24,7 @@
 #include <atomic>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -32,6 +33,7 @@
 
 #include "device.h"
 #include "end.h"
+#include "end_device.h"
 #include "super.h"
 #include "tables.h"
 
@@ -40,27 +42,6 @@ namespace pe {
 pe_status plan_create(const std::vector<LineHost>& lines, EndPlan* plan);
 
 namespace {
-
-__device__ __forceinline__ int local_add(int a, int b) {
-  return ((((a & 0x33) + (b & 0x33)) & 0x33) + (((a & 0xcc) + (b & 0xcc)) & 0xcc)) & 0xff;
-}
-__device__ __forceinline__ int dot4(const int a[4], const int b[4]) {
-  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
-}
-__device__ __forceinline__ void strides4(const int s[4], int st[4]) {
-  st[3] = 1; st[2] = s[3]; st[1] = s[2] * s[3]; st[0] = s[1] * s[2] * s[3];
-}
-__device__ __forceinline__ S4 ld(const uint64_t* p) {
-  const ulonglong2* q = reinterpret_cast<const ulonglong2*>(p);
-  const ulonglong2 a = q[0], b = q[1];
-  S4 r; r.w[0] = a.x; r.w[1] = a.y; r.w[2] = b.x; r.w[3] = b.y;
-  return r;
-}
-__device__ __forceinline__ void st(uint64_t* p, S4 v) {
-  ulonglong2* q = reinterpret_cast<ulonglong2*>(p);
-  q[0] = make_ulonglong2(v.w[0], v.w[1]);
-  q[1] = make_ulonglong2(v.w[2], v.w[3]);
-}
 
 enum { FH = 0, FV = 1, FLO = 2, FHI = 3, FAS = 4 };
 
@@ -100,36 +81,32 @@ __device__ __forceinline__ void g3(const NodeFields& F, int k, int a, int fa, in
 constexpr int kThreads = 256;
 constexpr int kParts = 4096 / kThreads;
 
+// One output block of 4096 nodes, computed by 16 workgroups (`part` = which 256 nodes).
 // S35: blocks of 35-stone sections, whose one move fills the board (no child data).
+// B.line[d] indexes `lines` for every open dimension d (-1: closed or not computed).
 template <bool S35>
-__global__ __launch_bounds__(kThreads)
-void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __restrict__ lines,
-                         const SecDev* __restrict__ secs, const uint64_t* __restrict__ fields,
-                         const uint16_t* __restrict__ movetab, const uint8_t* __restrict__ refl,
-                         int nblocks, int blocks_per_xcd) {
-  // Workgroups are dispatched to the 8 XCDs round-robin: workgroup i runs on XCD i % 8.
-  // XCD x takes the contiguous run of blocks [x * blocks_per_xcd, ...), so the blocks that
-  // share child lines (neighbours along the last axes) and the two halves of a 128-byte line
-  // meet in the same L2.
-  const int xcd = blockIdx.x & 7, r = blockIdx.x >> 3;
-  const int blk = xcd * blocks_per_xcd + r / kParts, part = r % kParts;
-  if (r / kParts >= blocks_per_xcd || blk >= nblocks) return;
-  const BlockDev& B = blocks[blk];
-  const SecDev& S = secs[B.sec];
+__device__ __forceinline__ void fused_block(const BlockDev& B, const SecDev& S, const LineDev* __restrict__ lines,
+                                            const uint64_t* __restrict__ fields,
+                                            const uint16_t* __restrict__ movetab, const uint8_t* __restrict__ refl,
+                                            int part) {
   // wave w of the block holds (loc0, loc1) = (w / 8, w % 8) and its 64 lanes (loc2, loc3):
   // quadrants 0 and 1 are the same for the whole wave, so their table reads are scalar
   // loads and their partial products scalar operations.  Lanes outside a partial edge
   // block idle.
   const int wave = part * (kThreads / 64) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  const int bs0 = min(8, S.rmin_size[0] - 8 * B.b[0]), bs1 = min(8, S.rmin_size[1] - 8 * B.b[1]);
-  const int bs2 = min(8, S.rmin_size[2] - 8 * B.b[2]), bs3 = min(8, S.rmin_size[3] - 8 * B.b[3]);
+  // the block record's fields as plain int arrays (indexed by the dimension loop below)
+  const int bb[4] = {B.b[0], B.b[1], B.b[2], B.b[3]};
+  const int bline[4] = {B.line[0], B.line[1], B.line[2], B.line[3]};
+  const int bs0 = min(8, S.rmin_size[0] - 8 * bb[0]), bs1 = min(8, S.rmin_size[1] - 8 * bb[1]);
+  const int bs2 = min(8, S.rmin_size[2] - 8 * bb[2]), bs3 = min(8, S.rmin_size[3] - 8 * bb[3]);
   // the block's field entries of quadrants 2 and 3 (eight each) staged in LDS
   __shared__ uint64_t sf[2][8][60];
   for (int i = threadIdx.x; i < 2 * 8 * 60; i += kThreads) {
     const int q = 2 + i / 480, r = (i / 60) % 8, w = i % 60;
-    const int bsq = q == 2 ? bs2 : bs3;
-    sf[q - 2][r][w] = r < bsq ? fields[4 * int64_t(S.field_off[q]) + 60 * int64_t(8 * B.b[q] + r) + w] : 0;
+    const int bsq = q == 2 ? bs2 : bs3, bq = q == 2 ? bb[2] : bb[3];
+    const int64_t fo = q == 2 ? S.field_off[2] : S.field_off[3];
+    sf[q - 2][r][w] = r < bsq ? fields[4 * fo + 60 * int64_t(8 * bq + r) + w] : 0;
   }
   __syncthreads();
   int loc[4] = {wave >> 3, wave & 7, lane >> 3, lane & 7};
@@ -142,8 +119,8 @@ void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __r
   S4 t0 = s4_zero(), t1 = s4_zero();
 #pragma unroll
   for (int d = 0; d < 4; d++) {
-    if (S35 || B.line[d] < 0) continue;
-    const LineDev& L = lines[B.line[d]];
+    if (S35 || bline[d] < 0) continue;
+    const LineDev& L = lines[bline[d]];
     const int cd = L.child_dim;
     int base = 0, base_last = 0, local_sym = 0;
     const uint8_t* syms = refl + L.refl_base;
@@ -155,7 +132,7 @@ void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __r
       base_last += L.ax_stride_last[a] * x;
       if (L.reflected) local_sym |= syms[L.ax_rs[a] + loc[a]];
     }
-    const uint16_t* row = movetab + S.move_off[d] + 16 * (8 * B.b[d] + loc[d]);
+    const uint16_t* row = movetab + S.move_off[d] + 16 * (8 * bb[d] + loc[d]);
     const int nm = S.nmoves[d];
     const int last = L.child_length - 1;
     const uint64_t* in = L.input + 8 * L.in_base;
@@ -196,7 +173,7 @@ void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __r
   NodeFields F;
 #pragma unroll
   for (int a = 0; a < 2; a++)
-    F.glob[a] = fields + 4 * int64_t(S.field_off[a]) + 60 * int64_t(8 * B.b[a] + loc[a]);
+    F.glob[a] = fields + 4 * int64_t(S.field_off[a]) + 60 * int64_t(8 * bb[a] + loc[a]);
   F.lds[0] = sf[0][loc[2]];
   F.lds[1] = sf[1][loc[3]];
   uint64_t A[3] = {0, 0, 0}, Bv[3] = {0, 0, 0}, Cv[3] = {0, 0, 0};
@@ -257,6 +234,49 @@ void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __r
   }
   st(o, o0);
   st(o + 4, o1);
+}
+
+template <bool S35>
+__global__ __launch_bounds__(kThreads)
+void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __restrict__ lines,
+                         const SecDev* __restrict__ secs, const uint64_t* __restrict__ fields,
+                         const uint16_t* __restrict__ movetab, const uint8_t* __restrict__ refl,
+                         int nblocks, int blocks_per_xcd) {
+  // Workgroups are dispatched to the 8 XCDs round-robin: workgroup i runs on XCD i % 8.
+  // XCD x takes the contiguous run of blocks [x * blocks_per_xcd, ...), so the blocks that
+  // share child lines (neighbours along the last axes) and the two halves of a 128-byte line
+  // meet in the same L2.
+  const int xcd = blockIdx.x & 7, r = blockIdx.x >> 3;
+  const int blk = xcd * blocks_per_xcd + r / kParts, part = r % kParts;
+  if (r / kParts >= blocks_per_xcd || blk >= nblocks) return;
+  const BlockDev& B = blocks[blk];
+  fused_block<S35>(B, secs[B.sec], lines, fields, movetab, refl, part);
+}
+
+// One block line passed by value (no device descriptors): the line and its section record.
+struct LineLaunch {
+  LineDev line;
+  SecDev sec;
+};
+
+// Workgroup i computes part i % 16 of block i / 16 of the line; output block k at node
+// k * block_stride, child block k at node k * (block_stride + 1) (the reference's buffers).
+template <bool S35>
+__global__ __launch_bounds__(kThreads)
+void fused_line_kernel(const LineLaunch a, const uint64_t* __restrict__ fields,
+                       const uint16_t* __restrict__ movetab, const uint8_t* __restrict__ refl) {
+  const int k = int(blockIdx.x) / kParts, part = int(blockIdx.x) % kParts;
+  const int dim = a.line.dim;
+  BlockDev B;
+  B.sec = 0;
+#pragma unroll
+  for (int x = 0; x < 4; x++) {
+    B.b[x] = x == dim ? k : a.line.first_block[x];
+    B.line[x] = x == dim ? 0 : -1;
+  }
+  B.accumulate = a.line.accumulate;
+  B.out = a.line.output + 8 * int64_t(k) * a.line.block_stride;
+  fused_block<S35>(B, a.sec, &a.line, fields, movetab, refl, part);
 }
 
 int64_t slab_index(const pe_slab& s, const int b[4]) {
@@ -336,14 +356,13 @@ static pe_status slab_lines(const pe_slab& out, int nch, const pe_slab* ch, std:
   return PE_OK;
 }
 
-// Pack blocks, lines (with their child offsets and reflection symmetries), sections, field
-// and move tables into one device allocation.
+// Pack blocks, lines (with their child offsets and reflection symmetries), sections and
+// move tables into one device allocation (field tables come from the device's field pool).
 static pe_status pack_fused_plan(const std::vector<BlockDev>& blocks, const std::vector<LineHost>& lines,
-                                 const std::vector<SecDev>& secs, const std::vector<uint64_t>& fields,
-                                 const std::vector<uint16_t>& movetab, int64_t nodes, FusedPlan* plan,
-                                 std::string* err) {
+                                 const std::vector<SecDev>& secs, const std::vector<uint16_t>& movetab,
+                                 int64_t nodes, FusedPlan* plan, std::string* err) {
   if (blocks.size() > size_t(0x7fffffff) / kParts) { *err = "fused plan too large"; return PE_ERR_CAPACITY; }
-  // blob: blocks | lines | sections | fields | moves | offsets | symmetries
+  // blob: blocks | lines | sections | moves | offsets | symmetries
   std::vector<LineDev> devs(lines.size());
   std::vector<int64_t> offs;
   std::vector<uint8_t> refl;
@@ -368,8 +387,7 @@ static pe_status pack_fused_plan(const std::vector<BlockDev>& blocks, const std:
   auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
   const size_t o_lines = up(sizeof(BlockDev) * blocks.size());
   const size_t o_secs = o_lines + up(sizeof(LineDev) * devs.size());
-  const size_t o_fields = o_secs + up(sizeof(SecDev) * secs.size());
-  const size_t o_moves = o_fields + up(sizeof(uint64_t) * fields.size());
+  const size_t o_moves = o_secs + up(sizeof(SecDev) * secs.size());
   const size_t o_offs = o_moves + up(sizeof(uint16_t) * std::max<size_t>(movetab.size(), 1));
   const size_t o_refl = o_offs + up(sizeof(int64_t) * offs.size());
   const size_t bytes = o_refl + refl.size();
@@ -381,7 +399,6 @@ static pe_status pack_fused_plan(const std::vector<BlockDev>& blocks, const std:
   std::memcpy(blob.data(), blocks.data(), sizeof(BlockDev) * blocks.size());
   std::memcpy(blob.data() + o_lines, devs.data(), sizeof(LineDev) * devs.size());
   std::memcpy(blob.data() + o_secs, secs.data(), sizeof(SecDev) * secs.size());
-  std::memcpy(blob.data() + o_fields, fields.data(), sizeof(uint64_t) * fields.size());
   if (!movetab.empty()) std::memcpy(blob.data() + o_moves, movetab.data(), sizeof(uint16_t) * movetab.size());
   std::memcpy(blob.data() + o_offs, offs.data(), sizeof(int64_t) * offs.size());
   std::memcpy(blob.data() + o_refl, refl.data(), refl.size());
@@ -401,7 +418,6 @@ static pe_status pack_fused_plan(const std::vector<BlockDev>& blocks, const std:
   plan->nblocks = int(blocks.size());
   plan->off_lines = o_lines;
   plan->off_secs = o_secs;
-  plan->off_fields = o_fields;
   plan->off_moves = o_moves;
   plan->off_refl = o_refl;
   plan->output_nodes = nodes;
@@ -413,7 +429,6 @@ static pe_status pack_fused_plan(const std::vector<BlockDev>& blocks, const std:
 struct SlabBuild {
   std::vector<LineHost> lines;
   SecDev sec;
-  std::vector<uint64_t> fields;
   std::vector<uint16_t> movetab;
   std::vector<BlockDev> blocks;
   int64_t nodes = 0;
@@ -421,7 +436,7 @@ struct SlabBuild {
   std::string err;
 };
 
-static void build_slab(const pe_slab& out, int nch, const pe_slab* ch, SlabBuild* r) {
+static void build_slab_body(const pe_slab& out, int nch, const pe_slab* ch, SlabBuild* r) {
   const auto& Tb = tables();
   if (!out.data) { r->err = "output slab without memory"; r->status = PE_ERR_INVALID; return; }
   std::map<std::tuple<int, int, int, int>, int> index;
@@ -433,13 +448,13 @@ static void build_slab(const pe_slab& out, int nch, const pe_slab* ch, SlabBuild
   SecDev& sec = r->sec;
   std::memset(&sec, 0, sizeof sec);
   sec.turn = s.sum() & 1;
-  section_field_tables(s, &r->fields, sec.field_off);
   r->status = section_move_tables(s, &r->movetab, sec.move_off, sec.nmoves, &r->err);
   if (r->status != PE_OK) return;
   for (int a = 0; a < 4; a++) {
     const int b = rmin_bucket(s.c[a][0], s.c[a][1]);
     sec.rmin_off[a] = Tb.rmin_offsets[b];
     sec.rmin_size[a] = Tb.rmin_offsets[b + 1] - Tb.rmin_offsets[b];
+    sec.field_off[a] = field_pool_off(a, sec.turn, 1, s.c[a][0], s.c[a][1]);
   }
   int b[4];
   for (b[0] = out.lo[0]; b[0] < out.hi[0]; b[0]++)
@@ -462,6 +477,20 @@ static void build_slab(const pe_slab& out, int nch, const pe_slab* ch, SlabBuild
           r->blocks.push_back(B);
           r->nodes += n;
         }
+}
+
+// Worker-thread entry: no exception leaves the thread (a host allocation failure or a bad
+// index becomes the slab's error status).
+static void build_slab(const pe_slab& out, int nch, const pe_slab* ch, SlabBuild* r) {
+  try {
+    build_slab_body(out, nch, ch, r);
+  } catch (const std::bad_alloc&) {
+    r->status = PE_ERR_CAPACITY;
+    r->err = "host memory exhausted while building a fused plan";
+  } catch (const std::exception& e) {
+    r->status = PE_ERR_INVALID;
+    r->err = std::string("fused plan: ") + e.what();
+  }
 }
 
 pe_status fused_plan_create(int nout, const pe_slab* outs, int nch, const pe_slab* ch, FusedPlan* plan,
@@ -499,18 +528,14 @@ pe_status fused_plan_create(int nout, const pe_slab* outs, int nch, const pe_sla
   for (auto& th : pool) th.join();
   std::vector<LineHost> lines;
   std::vector<SecDev> secs;
-  std::vector<uint64_t> fields;
   std::vector<uint16_t> movetab;
   std::vector<BlockDev> blocks;
   int64_t nodes = 0;
   for (SlabBuild& r : builds) {
     if (r.status != PE_OK) { *err = r.err; return r.status; }
     SecDev sec = r.sec;
-    for (int a = 0; a < 4; a++) {
-      sec.field_off[a] += int32_t(fields.size() / 4);
+    for (int a = 0; a < 4; a++)
       if (sec.move_off[a] >= 0) sec.move_off[a] += int32_t(movetab.size());
-    }
-    fields.insert(fields.end(), r.fields.begin(), r.fields.end());
     movetab.insert(movetab.end(), r.movetab.begin(), r.movetab.end());
     const int line0 = int(lines.size());
     for (auto& h : r.lines) lines.push_back(std::move(h));
@@ -524,11 +549,10 @@ pe_status fused_plan_create(int nout, const pe_slab* outs, int nch, const pe_sla
     secs.push_back(sec);
     std::vector<BlockDev>().swap(r.blocks);
   }
-  return pack_fused_plan(blocks, lines, secs, fields, movetab, nodes, plan, err);
+  return pack_fused_plan(blocks, lines, secs, movetab, nodes, plan, err);
 }
 
 bool fused_line_eligible(const LineHost& h) {
-  if (h.dev.slice35) return false;
   if (h.dev.i_begin != 0 || h.dev.i_end != h.dev.rmin_size[h.dev.dim]) return false;
   const std::vector<int64_t>& io = h.in_off;
   for (size_t k = 2; k < io.size(); k++)
@@ -537,22 +561,24 @@ bool fused_line_eligible(const LineHost& h) {
 }
 
 // Each job's blocks become fused blocks with only the job's dimension open: the section's
-// field table with the move fields (Fm) of the other quadrants cleared, and only that
+// fields from the pool with the move fields (Fm) of the other quadrants zero, and only that
 // dimension's move table, so a block computes exactly the line's contribution
-// (compute.cc:219-371 for one dimension); accumulate jobs OR it into the output.
+// (compute.cc:219-371 for one dimension); accumulate jobs OR it into the output.  The jobs of
+// one plan are all 35-stone lines (no child data) or all have at most 34 stones.
 pe_status fused_line_plan_create(const std::vector<LineHost>& jobs, FusedPlan* plan, std::string* err) {
   DevTables T;
   if (device_tables(&T, err) != PE_OK) return PE_ERR_DEVICE;
   const auto& Tb = tables();
   std::vector<LineHost> lines;
   std::vector<SecDev> secs;
-  std::vector<uint64_t> fields;
   std::vector<uint16_t> movetab;
   std::vector<BlockDev> blocks;
   std::map<std::pair<uint64_t, int>, int> sec_index;   // (section sig, dimension) -> SecDev
   int64_t nodes = 0;
+  plan->s35 = !jobs.empty() && jobs[0].dev.slice35;
   for (const LineHost& job : jobs) {
     if (!fused_line_eligible(job)) { *err = "internal: line job not eligible for the fused kernel"; return PE_ERR_INVALID; }
+    if (bool(job.dev.slice35) != plan->s35) { *err = "internal: fused line plan mixes 35-stone lines with others"; return PE_ERR_INVALID; }
     const int d = job.dev.dim;
     Section own;   // the line's own section
     std::memcpy(own.c, job.section.counts, 8);
@@ -562,28 +588,18 @@ pe_status fused_line_plan_create(const std::vector<LineHost>& jobs, FusedPlan* p
       SecDev sec;
       std::memset(&sec, 0, sizeof sec);
       sec.turn = own.sum() & 1;
-      std::vector<uint64_t> f;
-      section_field_tables(own, &f, sec.field_off);
-      // clear the move fields of every quadrant but d: only d's moves are this line's
-      for (int q = 0; q < 4; q++) {
-        if (q == d) continue;
-        const int n = rmin_count(own.c[q][0], own.c[q][1]);
-        for (int i = 0; i < n; i++)
-          for (int k = 0; k < 4; k++)
-            for (int fld = 0; fld < 5; fld++) f[size_t(4 * sec.field_off[q] + 60 * i + 15 * k + 10 + fld)] = 0;
-      }
       std::vector<uint16_t> mt;
-      pe_status st = section_move_tables(own, &mt, sec.move_off, sec.nmoves, err);
+      pe_status st = section_move_tables(own, &mt, sec.move_off, sec.nmoves, err, d);
       if (st != PE_OK) return st;
       for (int a = 0; a < 4; a++) {
-        sec.field_off[a] += int32_t(fields.size() / 4);
+        // only d's moves are this line's
+        sec.field_off[a] = field_pool_off(a, sec.turn, a == d, own.c[a][0], own.c[a][1]);
         if (a != d) { sec.move_off[a] = -1; sec.nmoves[a] = 0; }
-        else sec.move_off[a] += int32_t(movetab.size());
+        else if (sec.move_off[a] >= 0) sec.move_off[a] += int32_t(movetab.size());
         const int b = rmin_bucket(own.c[a][0], own.c[a][1]);
         sec.rmin_off[a] = Tb.rmin_offsets[b];
         sec.rmin_size[a] = Tb.rmin_offsets[b + 1] - Tb.rmin_offsets[b];
       }
-      fields.insert(fields.end(), f.begin(), f.end());
       movetab.insert(movetab.end(), mt.begin(), mt.end());
       it = sec_index.emplace(key, int(secs.size())).first;
       secs.push_back(sec);
@@ -609,7 +625,7 @@ pe_status fused_line_plan_create(const std::vector<LineHost>& jobs, FusedPlan* p
     }
     lines.push_back(std::move(h));
   }
-  return pack_fused_plan(blocks, lines, secs, fields, movetab, nodes, plan, err);
+  return pack_fused_plan(blocks, lines, secs, movetab, nodes, plan, err);
 }
 
 pe_status fused_plan_run(const FusedPlan& plan, hipStream_t stream) {
@@ -624,8 +640,7 @@ pe_status fused_plan_run(const FusedPlan& plan, hipStream_t stream) {
   hipLaunchKernelGGL(plan.s35 ? fused_blocks_kernel<true> : fused_blocks_kernel<false>, dim3(unsigned(grid)),
                      dim3(kThreads), 0, stream,
                      reinterpret_cast<const BlockDev*>(base), reinterpret_cast<const LineDev*>(base + plan.off_lines),
-                     reinterpret_cast<const SecDev*>(base + plan.off_secs),
-                     reinterpret_cast<const uint64_t*>(base + plan.off_fields),
+                     reinterpret_cast<const SecDev*>(base + plan.off_secs), T.fields,
                      reinterpret_cast<const uint16_t*>(base + plan.off_moves), base + plan.off_refl, plan.nblocks,
                      per_xcd);
   PE_HIP(hipGetLastError(), "fused_blocks_kernel launch");
@@ -678,6 +693,147 @@ pe_status slab_line_passes(const pe_slab& out, int nch, const pe_slab* ch, std::
     for (auto& g : group) (*passes)[pass].push_back(std::move(g));
     pass++;
   }
+  return PE_OK;
+}
+
+// ---- single block lines launched by value (the asynchronous schedule_compute_line form) ----
+
+namespace {
+
+// Per (device, section, dimension): the section record of a one-dimension fused block, that
+// dimension's move rows and, for a child standardized through a reflection, the reflection
+// symmetry of every rotation-minimal quadrant of each child axis (compute.cc:96-122; a
+// line's all_reflection_symmetries are the entries of its child blocks), in device memory.
+// Built on first use and kept until the process ends (32 bytes per rotation-minimal index
+// of the line's quadrant plus at most 1,680 bytes of symmetries: a few KB per section and
+// dimension); the entries are immutable, so launches in flight may read them while other
+// threads add entries.
+struct LineTabs {
+  SecDev sec;
+  const uint16_t* moves = nullptr;
+  const uint8_t* refl = nullptr;
+  int32_t refl_off[4] = {0, 0, 0, 0};   // child axis c's symmetries start at refl + refl_off[c]
+};
+struct LineCache {
+  std::mutex mu;
+  std::map<std::pair<uint64_t, int>, LineTabs> tabs;
+  uint8_t* chunk = nullptr;      // bump allocation of move rows
+  size_t left = 0;
+  hipStream_t upload = nullptr;  // non-blocking: uploads do not wait for the caller's work
+};
+constexpr int kMaxLineDevices = 64;
+LineCache g_line_cache[kMaxLineDevices];
+
+pe_status line_tabs(const LineHost& h, LineTabs* out, std::string* err) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxLineDevices) { *err = "no HIP device"; return PE_ERR_DEVICE; }
+  LineCache& C = g_line_cache[dev];
+  Section own;
+  std::memcpy(own.c, h.section.counts, 8);
+  const int d = h.dev.dim;
+  const auto key = std::make_pair(own.sig(), d);
+  std::lock_guard<std::mutex> lock(C.mu);
+  const auto it = C.tabs.find(key);
+  if (it != C.tabs.end()) { *out = it->second; return PE_OK; }
+  const auto& Tb = tables();
+  LineTabs t;
+  SecDev& sec = t.sec;
+  std::memset(&sec, 0, sizeof sec);
+  sec.turn = own.sum() & 1;
+  std::vector<uint16_t> mt;
+  const pe_status st = section_move_tables(own, &mt, sec.move_off, sec.nmoves, err, d);
+  if (st != PE_OK) return st;
+  const int32_t rows_off = sec.move_off[d];
+  for (int a = 0; a < 4; a++) {
+    sec.field_off[a] = field_pool_off(a, sec.turn, a == d, own.c[a][0], own.c[a][1]);
+    const int b = rmin_bucket(own.c[a][0], own.c[a][1]);
+    sec.rmin_off[a] = Tb.rmin_offsets[b];
+    sec.rmin_size[a] = Tb.rmin_offsets[b + 1] - Tb.rmin_offsets[b];
+    if (a != d) { sec.move_off[a] = -1; sec.nmoves[a] = 0; }
+  }
+  // reflection symmetries per child axis (zero unless the child is reflected)
+  std::vector<uint8_t> refl;
+  if (rows_off >= 0 && (h.info.section_transform & 4)) {
+    for (int c = 0; c < 4; c++) {
+      const int b = rmin_bucket(h.info.standard_child.counts[c][0], h.info.standard_child.counts[c][1]);
+      t.refl_off[c] = int32_t(refl.size());
+      for (int i = Tb.rmin_offsets[b]; i < Tb.rmin_offsets[b + 1]; i++) {
+        const uint16_t q = Tb.rmin_flat[i];
+        const uint16_t r = uint16_t(Tb.pack[Tb.refl[Tb.unpack[q][0]]] + 2 * Tb.pack[Tb.refl[Tb.unpack[q][1]]]);
+        refl.push_back(uint8_t((Tb.rmin_inverse[r] & 3) << (2 * c)));
+      }
+    }
+  }
+  if (rows_off >= 0) {   // lines with child data: upload d's move rows and the symmetries
+    const size_t rows = sizeof(uint16_t) * 16 * size_t(sec.rmin_size[d]);
+    const size_t bytes = rows + refl.size();
+    const size_t need = (bytes + 255) & ~size_t(255);
+    if (need > C.left) {
+      const size_t chunk = std::max<size_t>(need, size_t(4) << 20);
+      void* p = nullptr;
+      if (hipMalloc(&p, chunk) != hipSuccess) { *err = "hipMalloc(line move tables) failed"; return PE_ERR_DEVICE; }
+      C.chunk = static_cast<uint8_t*>(p);
+      C.left = chunk;
+    }
+    if (!C.upload && hipStreamCreateWithFlags(&C.upload, hipStreamNonBlocking) != hipSuccess) {
+      C.upload = nullptr;
+      *err = "hipStreamCreate(line tables) failed";
+      return PE_ERR_DEVICE;
+    }
+    std::vector<uint8_t> blob(bytes);
+    std::memcpy(blob.data(), mt.data() + rows_off, rows);
+    if (!refl.empty()) std::memcpy(blob.data() + rows, refl.data(), refl.size());
+    if (hipMemcpyAsync(C.chunk, blob.data(), bytes, hipMemcpyHostToDevice, C.upload) != hipSuccess ||
+        hipStreamSynchronize(C.upload) != hipSuccess) {
+      *err = "hipMemcpy(line move tables) failed";
+      return PE_ERR_DEVICE;
+    }
+    t.moves = reinterpret_cast<const uint16_t*>(C.chunk);
+    t.refl = C.chunk + rows;
+    sec.move_off[d] = 0;
+    C.chunk += need;
+    C.left -= need;
+  }
+  C.tabs.emplace(key, t);
+  *out = t;
+  return PE_OK;
+}
+
+}  // namespace
+
+pe_status fused_line_launch(const LineHost& h, const void* input, void* output, int accumulate, hipStream_t stream,
+                            std::string* err) {
+  DevTables T;
+  if (device_tables(&T, err) != PE_OK) return PE_ERR_DEVICE;
+  LineTabs t;
+  const pe_status st = line_tabs(h, &t, err);
+  if (st != PE_OK) return st;
+  LineLaunch a;
+  std::memset(&a, 0, sizeof a);
+  a.line = h.dev;
+  a.line.input = static_cast<const uint64_t*>(input);
+  a.line.output = static_cast<uint64_t*>(output);
+  a.line.accumulate = accumulate ? 1 : 0;
+  a.line.in_base = 0;
+  a.line.in_kstride = int64_t(h.dev.block_stride) + 1;   // pentago/end/compute.cc:134-139
+  a.line.in_off = nullptr;
+  a.line.out_off = nullptr;
+  a.line.refl_base = 0;
+  // the symmetries of cross axis x are those of child axis c = perm^-1(x) from the line's
+  // child block on, as all_reflection_symmetries holds them
+  for (int x = 0; x < 4; x++) {
+    a.line.ax_rs[x] = 0;
+    if (x == h.dev.dim || !t.refl) continue;
+    int c = 0;
+    while (h.dev.perm[c] != x) c++;
+    a.line.ax_rs[x] = t.refl_off[c] + 8 * h.dev.first_child_block[c];
+  }
+  a.sec = t.sec;
+  const unsigned grid = unsigned(h.dev.length) * kParts;
+  hipLaunchKernelGGL(h.dev.slice35 ? fused_line_kernel<true> : fused_line_kernel<false>, dim3(grid), dim3(kThreads), 0,
+                     stream, a, T.fields, t.moves, t.refl);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) { *err = std::string("fused_line_kernel launch: ") + hipGetErrorString(e); return PE_ERR_DEVICE; }
   return PE_OK;
 }
 

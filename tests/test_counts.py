@@ -182,3 +182,19 @@ def test_slice_driver_grouping():
     gs = S.plan_groups(s32, budget, False)
     assert all(P.slice_bytes(g) <= budget for g in gs)
     assert all(P.slice_bytes(g + h[:1]) > budget for g, h in zip(gs, gs[1:]))
+
+
+def test_slice24_enumeration_totals():
+    """pentago/end/end_test.cc:110-118: all 24-stone sections hold 1,921,672,470,396 nodes in
+    500,235,319 blocks of 8^4 and 95,263,785 block lines (lines only along dimensions whose
+    quadrant has an empty cell)."""
+    sections = P.descendent_sections(((0, 0),) * 4, 24)
+    nodes = blocks = lines = 0
+    for c in sections:
+        shape = P.section_shape(c)
+        nb = [(s + 7) // 8 for s in shape]
+        nodes += int(np.prod(shape, dtype=np.int64))
+        b = int(np.prod(nb, dtype=np.int64))
+        blocks += b
+        lines += sum(b // nb[d] for d in range(4) if sum(c[d]) < 9)
+    assert (nodes, blocks, lines) == (1921672470396, 500235319, 95263785)

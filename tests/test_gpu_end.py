@@ -109,6 +109,102 @@ def test_schedule_line_calls_wakeup_once():
     del cb
 
 
+def test_schedule_line_from_threads_is_asynchronous_and_wakes_once():
+    """schedule_compute_line's contract (pentago/end/compute.cc:359-370,394-404,
+    compute.h:84-86; called from pool threads, pentago/mpi/flow.cc:465-494): 8 host threads
+    issue 64 lines at once on their own streams, each call returns without waiting for its
+    line, wakeup fires exactly once per line after its output is complete, and every output
+    equals the oracle's."""
+    import threading
+    import time
+    nthreads, per_thread = 8, 8
+    jobs = []
+    for k in range(nthreads * per_thread):
+        counts, dim, where = CASES[k % len(CASES)]
+        pl, ol = make_case(counts, dim, where)
+        info = P.line_describe(pl)
+        inp = line_input(info, 5000 + k)
+        din = hip.Buffer(max(64, 64 * info.input_nodes))
+        if inp is not None:
+            din.upload(inp)
+        dout = hip.Buffer(64 * info.output_nodes)
+        dout.zero()
+        jobs.append((pl, ol, info, inp, din, dout))
+    streams = [hip.Stream() for _ in range(nthreads)]
+    lock = threading.Lock()
+    wakes = {}
+    done_at = {}
+
+    def wakeup(ctx):
+        with lock:
+            wakes[ctx] = wakes.get(ctx, 0) + 1
+            done_at.setdefault(ctx, time.perf_counter())
+
+    cb = P.WAKEUP(wakeup)
+    returned_at = {}
+    errors = []
+
+    def issue(t):
+        try:
+            for k in range(t, len(jobs), nthreads):
+                pl, _, _, _, din, dout = jobs[k]
+                st = P.lib().pe_end_schedule_line(C.byref(pl), din.ptr, dout.ptr, streams[t].ptr, cb, k + 1)
+                returned_at[k + 1] = time.perf_counter()
+                assert st == 0, P.lib().pe_last_error()
+        except Exception as e:  # reported by the main thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=issue, args=(t,)) for t in range(nthreads)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert not errors, errors
+    for s in streams:
+        s.synchronize()
+    deadline = time.time() + 20
+    while len(wakes) < len(jobs) and time.time() < deadline:
+        time.sleep(0.01)
+    time.sleep(0.2)   # a second wakeup of any line would land here
+    assert sorted(wakes) == list(range(1, len(jobs) + 1))
+    assert all(v == 1 for v in wakes.values()), wakes
+    # wakeups come after the calls returned (the calls did not wait for the lines)
+    assert all(done_at[c] >= returned_at[c] for c in wakes)
+    outs = [np.zeros(8 * j[2].output_nodes, dtype=np.uint64) for j in jobs]
+    refs = [np.zeros_like(o) for o in outs]
+    O.compute_lines_threaded([j[1] for j in jobs], [j[3] for j in jobs], refs, 16)
+    for k, (j, out, ref) in enumerate(zip(jobs, outs, refs)):
+        j[5].download(out)
+        assert np.array_equal(out, ref), (k, CASES[k % len(CASES)])
+    del cb
+
+
+def test_compute_line_accumulating_plan_jobs_slice35():
+    """35-stone whole-line jobs now run on the fused kernel's no-children form: a plan of
+    slice-35 lines (one of them OR-accumulating into prefilled output) equals the oracle."""
+    counts = ((5, 4), (4, 5), (4, 4), (5, 4))
+    shape = P.section_shape(counts)
+    jobs, refs, bufs = [], [], []
+    rng = np.random.default_rng(35)
+    for k, base in enumerate([(0, 0, 0), tuple((shape[a] - 1) // 8 for a in range(4) if a != 2)]):
+        pl, ol = P.make_line(counts, 2, base), O.make_line(counts, 2, base)
+        info = P.line_describe(pl)
+        ref = np.zeros(8 * info.output_nodes, dtype=np.uint64)
+        O.compute_lines_threaded([ol], [None], [ref], 4)
+        prefill = rng.integers(0, 2**63, size=ref.size, dtype=np.uint64) if k else np.zeros_like(ref)
+        out = hip.Buffer(64 * info.output_nodes)
+        out.upload(prefill)
+        jobs.append({"line": pl, "input": None, "output": out.ptr, "accumulate": k})
+        refs.append(ref | prefill)
+        bufs.append(out)
+    plan = P.Plan(jobs)
+    plan.run()
+    hip.synchronize()
+    for out, ref in zip(bufs, refs):
+        assert np.array_equal(out.download(np.zeros_like(ref)), ref)
+    plan.close()
+
+
 def test_seeded_fill_matches_host_generator():
     n = 5000
     buf = hip.Buffer(64 * n)

@@ -312,3 +312,46 @@ def test_end_line_brute_force_slice34():
         me, them = O.unpack_board(board, turn), O.unpack_board(board, 1 - turn)
         assert O.eval_node(std, idx) == O.brute_super(me, them)
     O.lib().po_eval_reset()
+
+
+def result_boards(s0, s1):
+    """The boards of midsolve_internal's results (pentago/mid/mid_test.cc:32-44): the root,
+    then the root with the next stone on each empty cell in increasing bit order."""
+    count = bin(s0 | s1).count("1")
+    boards = [(s0, s1)]
+    empty = 0x01ff01ff01ff01ff & ~(s0 | s1)
+    for bit in range(64):
+        if empty >> bit & 1:
+            after = [s0, s1]
+            after[count & 1] |= 1 << bit
+            boards.append(tuple(after))
+    return boards
+
+
+@pytest.mark.parametrize("stones,boards", [(35, 4), (34, 4), (33, 4), (32, 4), (31, 2), (30, 1)])
+def test_midsolve_internal_all_bits_brute_force(stones, boards):
+    """mid_test.cc:45-93: every one of the 128 bits of every (win, notlose) result of
+    midsolve_internal equals a brute-force game search of its board under that rotation;
+    the halfsuper of parity p sits at the rotations merge() gives parity p, the root at the
+    board's own parity (middle) and its children at the other."""
+    rng = np.random.default_rng(5554 + stones)
+    for trial in range(boards):
+        for middle in (0, 1):
+            while True:
+                s0, s1 = random_sides(rng, (stones + 1) // 2, stones // 2)
+                if not O.won(s0) and not O.won(s1):
+                    break
+            root = O.high_from_board(O.pack_board(s0, s1), middle)
+            results = O.midsolve_internal(root)[:37 - stones]
+            rboards = result_boards(s0, s1)
+            assert len(results) == len(rboards) == 37 - stones
+            for i, ((b0, b1), (win, notlose)) in enumerate(zip(rboards, results)):
+                assert win & ~notlose == 0, (stones, middle, i)
+                turn = bin(b0 | b1).count("1") & 1
+                me, them = (b1, b0) if turn else (b0, b1)
+                rparity = (i > 0) != bool(middle)
+                expand = (lambda h: O.merge(0, h)) if rparity else (lambda h: O.merge(h, 0))
+                known = expand(2**128 - 1)
+                correct_win, correct_notlose = O.brute_super(me, them)
+                assert (correct_win ^ expand(win)) & known == 0, (stones, middle, i, hex(b0), hex(b1))
+                assert (correct_notlose ^ expand(notlose)) & known == 0, (stones, middle, i, hex(b0), hex(b1))
