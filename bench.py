@@ -139,35 +139,103 @@ def build_boxes(P, hip, specs, mode=0):
     return outs, kids, plan, out_nodes, child_nodes
 
 
-def cpu_baseline(budget_s=15.0):
-    """The oracle (CPU port of compute_microline) on full block lines of the same section,
-    threaded one microline per job; superpositions/s = node-line rate / open dimensions."""
+def host_info():
+    """The host the CPU baselines run on: core counts and CPU model."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
+    return {"nproc": os.cpu_count(), "affinity": affinity, "cpu_model": model}
+
+
+def baseline_threads():
+    """Host threads for the all-core CPU figures: the cores this process may use, at most 16
+    (one GPU's share of the host's cores)."""
+    return max(1, min(host_info()["affinity"] or os.cpu_count() or 1, 16))
+
+
+def oracle_for_timing():
+    """The oracle library the CPU baselines time: the -O3 x86-64-v3 (AVX2) build of the same
+    C sources where the host supports it, else the test build (-O2)."""
     from oracle import oracle as O
-    from tests.seeded import seeded_nodes
-    threads = min(os.cpu_count() or 1, 16)
-    lines, ins, outs = [], [], []
-    nodes = 0
-    for d in range(4):
-        for base in [(5, 7, 9), (11, 3, 2)]:
-            line = O.make_line(SECTION, d, base)
-            info = O.line_describe(line)
-            lines.append(line)
-            ins.append(seeded_nodes(int(info.input_nodes), 99 + d))
-            outs.append(np.zeros(8 * info.output_nodes, dtype=np.uint64))
-            nodes += info.output_nodes
-    # calibrate with one line, then run as many full rounds as fit the budget
+    fast = O.fast_lib()
+    build = ("oracle C sources, gcc -O3 -march=x86-64-v3 (AVX2); portable C, not the reference's SSE build "
+             "(whose compute kernel is 1.81-2.25x its own 64-bit path, paper/pentago.tex:648)") if fast else \
+        "oracle C sources, gcc -O2 (host without AVX2); portable C, not the reference's SSE build"
+    return O, fast, build
+
+
+def timed_lines(O, L, lines, ins, outs, threads, budget_s):
+    """Node-line contributions per second of compute_lines_threaded over whole rounds that
+    fit the budget (at least one)."""
+    nodes = sum(int(O.line_describe(l).output_nodes) for l in lines)
     t0 = time.time()
-    O.compute_lines_threaded(lines[:1], ins[:1], outs[:1], threads)
+    O.compute_lines_threaded(lines, ins, outs, threads, L=L)
     one = time.time() - t0
-    rounds = max(1, int(budget_s / max(one * len(lines), 1e-3)))
+    rounds = max(1, int(budget_s / max(one, 1e-3)) - 1)
     t0 = time.time()
     for _ in range(rounds):
-        O.compute_lines_threaded(lines, ins, outs, threads)
-    dt = time.time() - t0
-    rate = rounds * nodes / dt          # node-line contributions per second
+        O.compute_lines_threaded(lines, ins, outs, threads, L=L)
+    dt = time.time() - t0 + one
+    return (rounds + 1) * nodes / dt, rounds + 1, dt
+
+
+def cpu_baseline(budget_s=15.0):
+    """The oracle's threaded port of compute_microline (one job per microline, as
+    pentago/end/compute.cc:378-404 schedules it) on full block lines of the headline section
+    (3,3)^4: all-core and single-thread superpositions/s (node-line rate / 4 open dimensions);
+    plus lines of the slice-30 section ((3,4),(4,4),(4,4),(4,3)) with the whole-section time
+    extrapolated from them."""
+    from tests.seeded import seeded_nodes
+    O, L, build = oracle_for_timing()
+    threads = baseline_threads()
+
+    def lines_of(section, bases, seed):
+        lines, ins, outs = [], [], []
+        for d in range(4):
+            if sum(section[d]) >= 9:
+                continue
+            for base in bases:
+                line = O.make_line(section, d, base)
+                info = O.line_describe(line)
+                lines.append(line)
+                ins.append(seeded_nodes(int(info.input_nodes), seed + d))
+                outs.append(np.zeros(8 * info.output_nodes, dtype=np.uint64))
+        return lines, ins, outs
+
+    lines, ins, outs = lines_of(SECTION, [(5, 7, 9), (11, 3, 2)], 99)
+    rate, rounds, dt = timed_lines(O, L, lines, ins, outs, threads, 0.5 * budget_s)
+    rate1, rounds1, dt1 = timed_lines(O, L, lines[::2], ins[::2], outs[::2], 1, 0.2 * budget_s)
+    # slice-30 section lines: 2 per dimension, whole-section time extrapolated
+    s30 = ((3, 4), (4, 4), (4, 4), (4, 3))
+    shape = O.section_shape(s30)
+    l30, i30, o30 = lines_of(s30, [(3, 9, 17), (7, 2, 13)], 123)
+    r30, rounds30, dt30 = timed_lines(O, L, l30, i30, o30, threads, 0.2 * budget_s)
+    section_nodes = int(np.prod(shape))
+    open_dims = sum(1 for d in range(4) if sum(s30[d]) < 9)
     return {"value": rate / 4, "unit": "superpositions/s", "cores": threads, "kind": "port",
-            "sample": "%d full block lines of section (3,3)^4 (2 per dimension, %d node contributions), "
-                      "%d rounds in %.1f s; rate / 4 open dimensions" % (len(lines), nodes, rounds, dt)}
+            "sample": "%d full block lines of section (3,3)^4 (2 per dimension), %d rounds in %.1f s on %d threads; "
+                      "superpositions/s = node-line rate / 4 open dimensions" % (len(lines), rounds, dt, threads),
+            "single_thread": {"value": rate1 / 4, "unit": "superpositions/s", "cores": 1,
+                              "sample": "%d of those lines, %d rounds in %.1f s" % (len(lines[::2]), rounds1, dt1)},
+            "section30": {"section": "(3,4)(4,4)(4,4)(4,3) (slice 30, 2,588,774,400 nodes)",
+                          "sample": "%d full block lines (2 per dimension), %d rounds in %.1f s on %d threads" % (
+                              len(l30), rounds30, dt30, threads),
+                          "superpositions_per_s": r30 / open_dims,
+                          "extrapolated_section_s": section_nodes * open_dims / r30,
+                          "extrapolated_section_s_single_thread": section_nodes * open_dims / (r30 * rate1 / rate),
+                          "note": "extrapolated from the sampled lines (the whole section is not run on the CPU); "
+                                  "single-thread: the all-core rate scaled by the (3,3)^4 single/all-core ratio"},
+            "host": host_info(), "build": build}
 
 
 def copy_probe(nbytes):
@@ -515,18 +583,38 @@ def serve_bench(P, rates=(200, 1000, 5000), seed=3, seconds=2.0):
 
 
 def mid_cpu_baseline(n=48, seed=2):
-    """The oracle's midsolve (one board per thread) on a sample of the batched workload."""
-    from oracle import oracle as O
+    """The oracle's midsolve (two passes per board, one board per thread) on a sample of the
+    batched workload (all-core), single-thread on the 26-stone board of the single-board
+    configuration and on 24..30-stone boards of the batch."""
     from pentago_amd.workloads import seeded_boards
-    threads = min(os.cpu_count() or 1, 16)
+    O, L, build = oracle_for_timing()
+    threads = baseline_threads()
+
+    def roots(bs):
+        return [O.High((O.C.c_uint64 * 2)(b.side[0], b.side[1]), b.ply) for b in bs]
+
     boards = seeded_boards(4096, seed, 18, 30)[:n]
-    roots = [O.High((O.C.c_uint64 * 2)(b.side[0], b.side[1]), b.ply) for b in boards]
     t0 = time.time()
-    O.midsolve_batch(roots, threads)
+    O.midsolve_batch(roots(boards), threads, L=L)
     dt = time.time() - t0
+    one = seeded_boards(1, 26, 26, 26)[0]
+    r1 = roots([one])
+    t0 = time.time()
+    reps = 0
+    while time.time() - t0 < 1.0:
+        O.midsolve_batch(r1, 1, L=L)
+        reps += 1
+    dt1 = time.time() - t0
+    small = [b for b in seeded_boards(256, seed, 24, 30)][:64]
+    t0 = time.time()
+    O.midsolve_batch(roots(small), 1, L=L)
+    dts = time.time() - t0
     return {"value": n / dt, "unit": "boards/s", "cores": threads, "kind": "port",
-            "sample": "first %d boards of the batched workload (18..30 stones), one board per thread, %.1f s"
-                      % (n, dt)}
+            "sample": "first %d boards of the batched workload (18..30 stones), one board per thread, %.1f s on %d "
+                      "threads" % (n, dt, threads),
+            "single_26_stones": {"boards_per_s": reps / dt1, "ms": 1e3 * dt1 / reps, "cores": 1},
+            "single_thread_24_30_stones": {"boards_per_s": len(small) / dts, "boards": len(small), "cores": 1},
+            "host": host_info(), "build": build}
 
 
 def main():

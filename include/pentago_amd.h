@@ -77,18 +77,24 @@ typedef struct {
  * blocks at node offset k*block_stride (compute.cc:147-152). */
 pe_status pe_line_describe(const pe_line* line, pe_line_info* info);
 
-/* Compute one block line on the device (asynchronous on `stream`).
- * Replaces schedule_compute_line(line_details_t&), pentago/end/compute.h:123. */
+/* Compute one block line on the device: one kernel launch on `stream`, returning without
+ * waiting for it.  Replaces schedule_compute_line(line_details_t&), pentago/end/compute.h:123
+ * (compute.cc:394-404).  Thread-safe (the reference calls it from pool threads,
+ * pentago/mpi/flow.cc:465-494).  The line's descriptor travels as kernel arguments, so a
+ * call takes no device memory; the per-(section, dimension) move and reflection tables are
+ * built on a section's first line and kept for the process (a few KB each).  d_input may be
+ * NULL only for 35-stone sections (no child data). */
 pe_status pe_end_compute_line(const pe_line* line, const void* d_input, void* d_output, void* stream);
 
-/* Same, then call wakeup(ctx) on a runtime thread once the output is complete: the
- * reference's completion callback line_details_t::wakeup (pentago/end/compute.h:84-86,
- * compute.cc:359-370). */
+/* Same, then call wakeup(ctx) exactly once, on a HIP runtime thread, once the output is
+ * complete: the reference's completion callback line_details_t::wakeup
+ * (pentago/end/compute.h:84-86, compute.cc:359-370).  wakeup must not call HIP. */
 pe_status pe_end_schedule_line(const pe_line* line, const void* d_input, void* d_output, void* stream,
                                void (*wakeup)(void* ctx), void* ctx);
 
-/* Many lines in one launch (lines of different sections and lengths may be mixed).
- * The batched form of pentago/end/compute.cc:394-404. */
+/* Many lines in one launch (lines of different sections and lengths may be mixed), built as
+ * a line plan and run; synchronous: returns after the lines are computed.  The batched form
+ * of pentago/end/compute.cc:394-404. */
 pe_status pe_end_compute_lines(int n, const pe_line* lines, const void* const* d_inputs,
                                void* const* d_outputs, void* stream);
 
@@ -259,7 +265,7 @@ typedef struct { uint64_t raw; int32_t value; int32_t pad; } pe_mid_value;
 pe_status pe_high_done(const pe_high_board* board, int* done, int* value);
 
 /* Workspace entries needed for boards with at least min_slice stones.
- * pentago/mid/midengine.cc:95-97 (entries are 32-byte (win, notlose) pairs here). */
+ * pentago/mid/midengine.cc:47-50 (entries are 32-byte (win, notlose) pairs here). */
 int64_t pe_midsolve_workspace_size(int min_slice);
 
 /* Solve one board (>= 18 stones) on the device: results[0..spots] as in
@@ -348,7 +354,7 @@ pe_status pe_mid_server_stats(pe_mid_server* server, pe_mid_server_info* stats);
 /* Finishes every submitted request, then releases the server. */
 pe_status pe_mid_server_destroy(pe_mid_server* server);
 
-/* Traverse precomputed results into (raw, value) pairs.  pentago/mid/midengine.cc:175-206 */
+/* Traverse precomputed results into (raw, value) pairs.  pentago/mid/midengine.cc:127-158 */
 pe_status pe_midsolve_traverse(const pe_high_board* board, const pe_halfsupers* results,
                                pe_mid_value* values, int* count);
 

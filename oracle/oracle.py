@@ -58,6 +58,73 @@ def build():
 
 
 _lib = None
+_fast = None
+# The same sources built with -O3 -march=x86-64-v3 (AVX2) for the CPU baseline timings.
+FAST_PATH = os.path.join(HERE, "liboracle_fast.so")
+
+
+def _load(path):
+    L = C.CDLL(path)
+    P = C.POINTER
+    sigs = {
+        "po_init": (None, []),
+        "po_table_hash": (C.c_int, [C.c_char_p, C.c_char_p]),
+        "po_rmax": (Super, [Super]),
+        "po_super_wins": (Super, [C.c_uint64]),
+        "po_won": (C.c_int, [C.c_uint64]),
+        "po_sym_mul": (Sym, [Sym, Sym]),
+        "po_sym_inverse": (Sym, [Sym]),
+        "po_transform_super": (Super, [Sym, Super]),
+        "po_transform_side": (C.c_uint64, [Sym, C.c_uint64]),
+        "po_transform_board": (C.c_uint64, [Sym, C.c_uint64]),
+        "po_meaningless": (C.c_int, [C.c_uint64, C.c_uint64]),
+        "po_super_meaningless": (Super, [C.c_uint64, C.c_uint64]),
+        "po_pack_board": (C.c_uint64, [C.c_uint64, C.c_uint64]),
+        "po_unpack_board": (C.c_uint64, [C.c_uint64, C.c_int]),
+        "po_rmin": (C.c_int, [C.c_int, C.c_int, P(P(C.c_uint16)), P(C.c_int)]),
+        "po_rmin_inverse": (C.c_uint16, [C.c_uint16]),
+        "po_section_shape": (C.c_int, [Section, P(C.c_int)]),
+        "po_section_standardize": (C.c_int, [Section, P(Section)]),
+        "po_line_describe": (C.c_int, [P(Line), P(LineInfo)]),
+        "po_compute_line": (C.c_int, [P(Line), C.c_void_p, C.c_void_p]),
+        "po_compute_microlines": (C.c_int, [P(Line), C.c_void_p, C.c_void_p, C.c_int, C.c_int]),
+        "po_compute_lines_threaded": (C.c_int, [C.c_int, P(Line), P(C.c_void_p), P(C.c_void_p), C.c_int]),
+        "po_eval_reset": (None, []),
+        "po_eval_node": (C.c_int, [Section, P(C.c_int), P(Super)]),
+        "po_popcounts_over_stabilizers": (C.c_int, [C.c_uint64, P(Super), P(C.c_uint16)]),
+        "po_section_counts": (C.c_int, [Section, P(C.c_uint64)]),
+        "po_interleave": (None, [C.c_int64, C.c_void_p]),
+        "po_uninterleave": (None, [C.c_int64, C.c_void_p]),
+        "po_brute_value": (C.c_int, [C.c_uint64, C.c_uint64]),
+        "po_brute_super": (None, [C.c_uint64, C.c_uint64, P(Super)]),
+        "po_half_rmax": (Half, [Half]),
+        "po_split": (None, [Super, P(Half)]),
+        "po_merge": (Super, [Half, Half]),
+        "po_halfsuper_wins": (Half, [C.c_uint64, C.c_int]),
+        "po_choose": (C.c_int, [C.c_int, C.c_int]),
+        "po_subset": (C.c_uint64, [C.c_int, C.c_int, C.c_int]),
+        "po_subset_mask": (C.c_uint32, [C.c_int, C.c_int, C.c_int]),
+        "po_subset_index": (C.c_int, [C.c_int, C.c_int, C.c_uint64]),
+        "po_bottleneck": (C.c_int, [C.c_int]),
+        "po_midsolve_workspace_size": (C.c_int, [C.c_int]),
+        "po_midsolve_internal": (C.c_int, [High, C.c_void_p, P(Halves)]),
+        "po_midsolve": (C.c_int, [High, C.c_void_p, P(C.c_uint64), P(C.c_int)]),
+        "po_midsolve_traverse": (C.c_int, [High, P(Halves), P(C.c_uint64), P(C.c_int)]),
+        "po_midsolve_batch": (C.c_int, [C.c_int, P(High), C.c_void_p, C.c_int]),
+        "po_high_from_board": (High, [C.c_uint64, C.c_int]),
+        "po_high_board": (C.c_uint64, [High]),
+        "po_high_raw": (C.c_uint64, [High]),
+        "po_high_place": (High, [High, C.c_int]),
+        "po_high_rotate": (High, [High, C.c_int, C.c_int]),
+        "po_high_done_and_value": (C.c_int, [High, P(C.c_int)]),
+        "po_high_count": (C.c_int, [High]),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    L.po_init()
+    return L
 
 
 def lib():
@@ -65,68 +132,22 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        P = C.POINTER
-        sigs = {
-            "po_init": (None, []),
-            "po_table_hash": (C.c_int, [C.c_char_p, C.c_char_p]),
-            "po_rmax": (Super, [Super]),
-            "po_super_wins": (Super, [C.c_uint64]),
-            "po_won": (C.c_int, [C.c_uint64]),
-            "po_sym_mul": (Sym, [Sym, Sym]),
-            "po_sym_inverse": (Sym, [Sym]),
-            "po_transform_super": (Super, [Sym, Super]),
-            "po_transform_side": (C.c_uint64, [Sym, C.c_uint64]),
-            "po_transform_board": (C.c_uint64, [Sym, C.c_uint64]),
-            "po_meaningless": (C.c_int, [C.c_uint64, C.c_uint64]),
-            "po_super_meaningless": (Super, [C.c_uint64, C.c_uint64]),
-            "po_pack_board": (C.c_uint64, [C.c_uint64, C.c_uint64]),
-            "po_unpack_board": (C.c_uint64, [C.c_uint64, C.c_int]),
-            "po_rmin": (C.c_int, [C.c_int, C.c_int, P(P(C.c_uint16)), P(C.c_int)]),
-            "po_rmin_inverse": (C.c_uint16, [C.c_uint16]),
-            "po_section_shape": (C.c_int, [Section, P(C.c_int)]),
-            "po_section_standardize": (C.c_int, [Section, P(Section)]),
-            "po_line_describe": (C.c_int, [P(Line), P(LineInfo)]),
-            "po_compute_line": (C.c_int, [P(Line), C.c_void_p, C.c_void_p]),
-            "po_compute_microlines": (C.c_int, [P(Line), C.c_void_p, C.c_void_p, C.c_int, C.c_int]),
-            "po_compute_lines_threaded": (C.c_int, [C.c_int, P(Line), P(C.c_void_p), P(C.c_void_p), C.c_int]),
-            "po_eval_reset": (None, []),
-            "po_eval_node": (C.c_int, [Section, P(C.c_int), P(Super)]),
-            "po_popcounts_over_stabilizers": (C.c_int, [C.c_uint64, P(Super), P(C.c_uint16)]),
-            "po_section_counts": (C.c_int, [Section, P(C.c_uint64)]),
-            "po_interleave": (None, [C.c_int64, C.c_void_p]),
-            "po_uninterleave": (None, [C.c_int64, C.c_void_p]),
-            "po_brute_value": (C.c_int, [C.c_uint64, C.c_uint64]),
-            "po_brute_super": (None, [C.c_uint64, C.c_uint64, P(Super)]),
-            "po_half_rmax": (Half, [Half]),
-            "po_split": (None, [Super, P(Half)]),
-            "po_merge": (Super, [Half, Half]),
-            "po_halfsuper_wins": (Half, [C.c_uint64, C.c_int]),
-            "po_choose": (C.c_int, [C.c_int, C.c_int]),
-            "po_subset": (C.c_uint64, [C.c_int, C.c_int, C.c_int]),
-            "po_subset_mask": (C.c_uint32, [C.c_int, C.c_int, C.c_int]),
-            "po_subset_index": (C.c_int, [C.c_int, C.c_int, C.c_uint64]),
-            "po_bottleneck": (C.c_int, [C.c_int]),
-            "po_midsolve_workspace_size": (C.c_int, [C.c_int]),
-            "po_midsolve_internal": (C.c_int, [High, C.c_void_p, P(Halves)]),
-            "po_midsolve": (C.c_int, [High, C.c_void_p, P(C.c_uint64), P(C.c_int)]),
-            "po_midsolve_traverse": (C.c_int, [High, P(Halves), P(C.c_uint64), P(C.c_int)]),
-            "po_midsolve_batch": (C.c_int, [C.c_int, P(High), C.c_void_p, C.c_int]),
-            "po_high_from_board": (High, [C.c_uint64, C.c_int]),
-            "po_high_board": (C.c_uint64, [High]),
-            "po_high_raw": (C.c_uint64, [High]),
-            "po_high_place": (High, [High, C.c_int]),
-            "po_high_rotate": (High, [High, C.c_int, C.c_int]),
-            "po_high_done_and_value": (C.c_int, [High, P(C.c_int)]),
-            "po_high_count": (C.c_int, [High]),
-        }
-        for name, (res, args) in sigs.items():
-            f = getattr(L, name)
-            f.restype = res
-            f.argtypes = args
-        L.po_init()
-        _lib = L
+        _lib = _load(LIB_PATH)
     return _lib
+
+
+def fast_lib():
+    """The optimized build (for timing), or None where it is missing or the host lacks AVX2."""
+    global _fast
+    if _fast is None:
+        flags = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                flags = next((l for l in f if l.startswith("flags")), "")
+        except OSError:
+            pass
+        _fast = _load(FAST_PATH) if os.path.exists(FAST_PATH) and "avx2" in flags.split() else False
+    return _fast or None
 
 
 # ---------------- conversions ----------------
@@ -282,12 +303,12 @@ def compute_microlines(line, inp, lo, hi, out=None):
     return out
 
 
-def compute_lines_threaded(lines, inputs, outputs, threads):
+def compute_lines_threaded(lines, inputs, outputs, threads, L=None):
     n = len(lines)
     arr = (Line * n)(*lines)
     ins = (C.c_void_p * n)(*[(i.ctypes.data if i is not None and i.size else None) for i in inputs])
     outs = (C.c_void_p * n)(*[o.ctypes.data for o in outputs])
-    assert lib().po_compute_lines_threaded(n, arr, ins, outs, threads) == 0
+    assert (L or lib()).po_compute_lines_threaded(n, arr, ins, outs, threads) == 0
 
 
 def eval_node(counts, index):
@@ -390,9 +411,9 @@ def midsolve_traverse(root, supers):
     return [(int(raws[i]), int(vals[i])) for i in range(n)]
 
 
-def midsolve_batch(roots, threads):
+def midsolve_batch(roots, threads, L=None):
     n = len(roots)
     arr = (High * n)(*roots)
     res = ((Halves * 19) * n)()
-    assert lib().po_midsolve_batch(n, arr, C.cast(res, C.c_void_p), threads) == 0
+    assert (L or lib()).po_midsolve_batch(n, arr, C.cast(res, C.c_void_p), threads) == 0
     return [[(half_to_int(r.win), half_to_int(r.notlose)) for r in res[i]] for i in range(n)]

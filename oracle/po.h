@@ -141,10 +141,10 @@ int po_subset_index(int n, int k, uint64_t set);
 int po_bottleneck(int spots);
 int po_midsolve_workspace_size(int min_slice);
 /* results: 1+18 (win, notlose) halfsuper pairs.  workspace: >= bottleneck entries.
- * pentago/mid/midengine.cc:141-173 */
+ * pentago/mid/midengine.cc:93-125 */
 int po_midsolve_internal(po_high root, po_half* workspace, po_halves results[19]);
 /* Full midsolve + traverse: writes up to 163 (raw, value) pairs; returns count.
- * pentago/mid/midengine.cc:175-217 */
+ * pentago/mid/midengine.cc:161-169 (midsolve) and :127-158 (midsolve_traverse) */
 int po_midsolve(po_high root, po_half* workspace, uint64_t* raws, int* values);
 int po_midsolve_traverse(po_high root, const po_halves* supers, uint64_t* raws, int* values);
 /* Threaded batch: one board per thread.  returns 0 on success. */

@@ -328,7 +328,7 @@ static void inner(const info_t* I, int n, const uint16_t* cs1ps, const uint64_t*
   workspace[gout.lo + (int64_t)s1*gout.ny + s0p] = hor(po_half_rmax(hnot(us)), w1.before);
 }
 
-/* midengine.cc:141-173 */
+/* midengine.cc:93-125 */
 int po_midsolve_internal(po_high root, po_half* workspace, po_halves results[19]) {
   const info_t I = make_info(root);
   if (I.spots > 18 || I.spots < 0) return -1;
@@ -368,7 +368,7 @@ int po_midsolve_internal(po_high root, po_half* workspace, po_halves results[19]
 /* midengine.h:28-30 */
 static int mid_value(const po_halves* r, int s) { return hget(r->win, s >> 1) + hget(r->notlose, s >> 1) - 1; }
 
-/* midengine.cc:175-206 */
+/* midengine.cc:127-158 */
 static int traverse(po_high board, const po_halves* supers, uint64_t* raws, int* values, int* count) {
   int value;
   int imm;

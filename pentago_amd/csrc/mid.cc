@@ -1,5 +1,5 @@
 // Midsolve host side: per-board constants and result traversal (own implementation of
-// pentago/mid/internal.h:37-69 and pentago/mid/midengine.cc:175-206).
+// pentago/mid/internal.h:37-69 and pentago/mid/midengine.cc:127-158).
 #include "mid.h"
 
 #include <algorithm>

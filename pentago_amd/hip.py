@@ -31,6 +31,8 @@ def hip():
             "hipEventSynchronize": [C.c_void_p],
             "hipEventElapsedTime": [P(C.c_float), C.c_void_p, C.c_void_p],
             "hipMemGetInfo": [P(C.c_size_t), P(C.c_size_t)],
+            "hipStreamWaitEvent": [C.c_void_p, C.c_void_p, C.c_uint],
+            "hipLaunchHostFunc": [C.c_void_p, C.c_void_p, C.c_void_p],
             "hipGetErrorString": [C.c_int],
         }
         for name, args in sig.items():

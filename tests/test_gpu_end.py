@@ -109,12 +109,15 @@ def test_schedule_line_calls_wakeup_once():
     del cb
 
 
+HOST_FN = C.CFUNCTYPE(None, C.c_void_p)
+
+
 def test_schedule_line_from_threads_is_asynchronous_and_wakes_once():
     """schedule_compute_line's contract (pentago/end/compute.cc:359-370,394-404,
     compute.h:84-86; called from pool threads, pentago/mpi/flow.cc:465-494): 8 host threads
-    issue 64 lines at once on their own streams, each call returns without waiting for its
-    line, wakeup fires exactly once per line after its output is complete, and every output
-    equals the oracle's."""
+    issue 64 lines at once on their own streams, every call returns while its stream is still
+    held back by a host-side gate (so no call waited for its line), wakeup fires exactly once
+    per line after its output is complete, and every output equals the oracle's."""
     import threading
     import time
     nthreads, per_thread = 8, 8
@@ -131,17 +134,23 @@ def test_schedule_line_from_threads_is_asynchronous_and_wakes_once():
         dout.zero()
         jobs.append((pl, ol, info, inp, din, dout))
     streams = [hip.Stream() for _ in range(nthreads)]
+    # the gate: a host function on its own stream that returns once the test opens it (or
+    # after 10 s at most); every line stream waits for the event recorded behind it
+    gate, gate_stream, gate_event = threading.Event(), hip.Stream(), hip.Event()
+    gate_fn = HOST_FN(lambda _: gate.wait(10.0))
+    hip.check(hip.hip().hipLaunchHostFunc(C.c_void_p(gate_stream.ptr), C.cast(gate_fn, C.c_void_p), None),
+              "hipLaunchHostFunc")
+    gate_event.record(gate_stream)
+    for s in streams:
+        hip.check(hip.hip().hipStreamWaitEvent(C.c_void_p(s.ptr), C.c_void_p(gate_event.ptr), 0), "hipStreamWaitEvent")
     lock = threading.Lock()
     wakes = {}
-    done_at = {}
 
     def wakeup(ctx):
         with lock:
             wakes[ctx] = wakes.get(ctx, 0) + 1
-            done_at.setdefault(ctx, time.perf_counter())
 
     cb = P.WAKEUP(wakeup)
-    returned_at = {}
     errors = []
 
     def issue(t):
@@ -149,17 +158,22 @@ def test_schedule_line_from_threads_is_asynchronous_and_wakes_once():
             for k in range(t, len(jobs), nthreads):
                 pl, _, _, _, din, dout = jobs[k]
                 st = P.lib().pe_end_schedule_line(C.byref(pl), din.ptr, dout.ptr, streams[t].ptr, cb, k + 1)
-                returned_at[k + 1] = time.perf_counter()
                 assert st == 0, P.lib().pe_last_error()
         except Exception as e:  # reported by the main thread
             errors.append(e)
 
-    threads = [threading.Thread(target=issue, args=(t,)) for t in range(nthreads)]
-    for th in threads:
-        th.start()
-    for th in threads:
-        th.join()
+    try:
+        threads = [threading.Thread(target=issue, args=(t,)) for t in range(nthreads)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+        with lock:
+            early = dict(wakes)
+    finally:
+        gate.set()
     assert not errors, errors
+    assert early == {}, "a line completed while its stream was held back"
     for s in streams:
         s.synchronize()
     deadline = time.time() + 20
@@ -168,15 +182,14 @@ def test_schedule_line_from_threads_is_asynchronous_and_wakes_once():
     time.sleep(0.2)   # a second wakeup of any line would land here
     assert sorted(wakes) == list(range(1, len(jobs) + 1))
     assert all(v == 1 for v in wakes.values()), wakes
-    # wakeups come after the calls returned (the calls did not wait for the lines)
-    assert all(done_at[c] >= returned_at[c] for c in wakes)
     outs = [np.zeros(8 * j[2].output_nodes, dtype=np.uint64) for j in jobs]
     refs = [np.zeros_like(o) for o in outs]
     O.compute_lines_threaded([j[1] for j in jobs], [j[3] for j in jobs], refs, 16)
     for k, (j, out, ref) in enumerate(zip(jobs, outs, refs)):
         j[5].download(out)
         assert np.array_equal(out, ref), (k, CASES[k % len(CASES)])
-    del cb
+    gate_stream.synchronize()
+    del cb, gate_fn
 
 
 def test_compute_line_accumulating_plan_jobs_slice35():
