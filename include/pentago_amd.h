@@ -272,6 +272,19 @@ int64_t pe_midsolve_workspace_size(int min_slice);
  * midsolve_internal, pentago/mid/midengine.h:40.  Synchronous. */
 pe_status pe_midsolve_internal(const pe_high_board* board, pe_halfsupers results[19]);
 
+/* pe_midsolve_internal for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Allocates nothing: the board's tables, workspace, results and launch
+ * descriptors live in a fixed device array of pe_midsolve_noalloc_scratch_bytes() (64 MiB)
+ * that every GPU the library runs on holds until the process ends, so boards with at most
+ * 15 empty cells (at least 21 stones) fit; larger boards return PE_ERR_CAPACITY.  pe_init
+ * must have run on the device (PE_ERR_INVALID otherwise).  Runs on `stream` and returns once
+ * results[0..18] are written (entries past spots zero), equal to pe_midsolve_internal's.
+ * Every call on a GPU shares that GPU's array: this library serializes the calls of one
+ * process, and calls from several processes that overlap on one GPU race. */
+pe_status pe_midsolve_internal_noalloc(const pe_high_board* board, pe_halfsupers results[19], void* stream);
+int64_t pe_midsolve_noalloc_scratch_bytes(void);
+
 /* Solve and traverse: values of the board, its children and (if not middle) grandchildren.
  * Returns the count through *count (<= 163).  pentago/mid/midengine.cc:171-179 (wasm
  * midsolve) and ios/shared/c_board.h:30-34 (board_midsolve). */

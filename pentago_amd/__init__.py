@@ -179,6 +179,8 @@ def lib():
             "pe_midsolve": (C.c_int, [P(HighBoard), P(MidValue), P(C.c_int)]),
             "pe_midsolve_batch": (C.c_int, [C.c_int, P(HighBoard), V, C.c_int, V]),
             "pe_midsolve_traverse": (C.c_int, [P(HighBoard), P(HalfSupers), P(MidValue), P(C.c_int)]),
+            "pe_midsolve_internal_noalloc": (C.c_int, [P(HighBoard), P(HalfSupers), V]),
+            "pe_midsolve_noalloc_scratch_bytes": (C.c_int64, []),
             "pe_high_done": (C.c_int, [P(HighBoard), P(C.c_int), P(C.c_int)]),
             "pe_mid_server_create": (C.c_int, [P(MidServerConfig), P(V)]),
             "pe_mid_server_submit": (C.c_int, [V, P(HighBoard), P(C.c_int64)]),
@@ -494,6 +496,15 @@ def midsolve_internal(board):
     """midsolve_internal (pentago/mid/midengine.h:40): list of (win, notlose) halfsupers."""
     res = (HalfSupers * 19)()
     _check(lib().pe_midsolve_internal(C.byref(board), res), "pe_midsolve_internal")
+    spots = 36 - ((board.ply + 1) >> 1)
+    return [_halves(res[i]) for i in range(spots + 1)]
+
+
+def midsolve_internal_noalloc(board, stream=None):
+    """pe_midsolve_internal_noalloc: midsolve_internal without any device allocation (boards
+    with at least 21 stones; the library's fixed device scratch array)."""
+    res = (HalfSupers * 19)()
+    _check(lib().pe_midsolve_internal_noalloc(C.byref(board), res, stream), "pe_midsolve_internal_noalloc")
     spots = 36 - ((board.ply + 1) >> 1)
     return [_halves(res[i]) for i in range(spots + 1)]
 

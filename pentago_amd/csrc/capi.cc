@@ -29,6 +29,8 @@ pe_status launch_fill_seeded(void* d_nodes, int64_t nodes, uint64_t seed, int64_
 pe_status mid_solve_levels(const std::vector<MidInfo>& infos, uint8_t* arena, std::vector<size_t>* result_offsets,
                            hipStream_t stream, DeviceBuffer* desc_cache);
 int mid_queue_blocks_per_cu();
+pe_status mid_solve_noalloc(const pe_high_board& board, pe_halfsupers* results, hipStream_t stream);
+int64_t mid_noalloc_scratch_bytes();
 
 namespace {
 thread_local std::string g_error;
@@ -102,6 +104,15 @@ pe_status device_tables(DevTables* out, std::string* err) {
   g_dev_ready[dev] = true;
   *out = D;
   return PE_OK;
+}
+
+bool device_tables_ready(DevTables* out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return false;
+  std::lock_guard<std::mutex> lock(g_dev_mutex);
+  if (!g_dev_ready[dev]) return false;
+  *out = g_dev[dev];
+  return true;
 }
 
 namespace {
@@ -770,6 +781,15 @@ pe_status pe_midsolve_internal(const pe_high_board* board, pe_halfsupers results
   return pe_midsolve_batch(1, board, r, 0, nullptr);
   PE_GUARD_END
 }
+
+pe_status pe_midsolve_internal_noalloc(const pe_high_board* board, pe_halfsupers results[19], void* stream) {
+  PE_GUARD_BEGIN
+  if (!board || !results) return fail(PE_ERR_INVALID, "null argument");
+  return mid_solve_noalloc(*board, results, static_cast<hipStream_t>(stream));
+  PE_GUARD_END
+}
+
+int64_t pe_midsolve_noalloc_scratch_bytes(void) { return mid_noalloc_scratch_bytes(); }
 
 pe_status pe_high_done(const pe_high_board* board, int* done, int* value) {
   PE_GUARD_BEGIN

@@ -26,6 +26,8 @@ struct DevTables {
 
 // Upload the tables to the current device (idempotent per device).
 pe_status device_tables(DevTables* out, std::string* err);
+// The current device's tables if they are already uploaded (never allocates).
+bool device_tables_ready(DevTables* out);
 
 void set_error(const std::string& e);
 

@@ -9,7 +9,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "mid_device.h"
@@ -220,6 +223,92 @@ pe_status mid_solve_levels(const std::vector<MidInfo>& infos, uint8_t* arena, st
     PE_HIP(hipGetLastError(), "mid_level_kernel launch");
   }
   if (!desc_cache) PE_HIP(hipFreeAsync(d, stream), "hipFreeAsync(mid descriptors)");
+  return PE_OK;
+}
+
+// ---- allocation-free single-board solve ----
+// For callers that cannot allocate device memory (code under HIP graph capture, tight
+// serving loops, harnesses that forbid hipMalloc); the reference has no counterpart (its
+// caller passes a workspace it owns).  The launch descriptors, the board's tables, its
+// workspace and its results live in `scratch`, a fixed array that every device the library
+// runs on holds until the process ends.  Host code never uses the array's name as an
+// address: each device's address comes from hipGetSymbolAddress, once per device.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu[kScratchDevices];      // one solve at a time per device in this process
+uint8_t* g_scratch_ptr[kScratchDevices];       // device address of `scratch` on each device
+
+// The descriptors of one board's launches: the board record, then the prefix sums the prep
+// and level kernels read (mid_solve_levels' layout for one board: [0, entries] per launch).
+struct NoAllocDesc {
+  MidBoardDev board;
+  int64_t prefix[2 * 21];
+};
+
+}  // namespace
+
+int64_t mid_noalloc_scratch_bytes() { return int64_t(kScratchBytes); }
+
+pe_status mid_solve_noalloc(const pe_high_board& hb, pe_halfsupers* results, hipStream_t stream) {
+  MidInfo I;
+  if (!make_mid_info(hb, &I)) { set_error("not a valid board with at least 18 stones"); return PE_ERR_INVALID; }
+  DevTables T;
+  if (!device_tables_ready(&T)) {
+    set_error("the device's tables are not uploaded: call pe_init on this device first (this entry point allocates nothing)");
+    return PE_ERR_INVALID;
+  }
+  const size_t desc_bytes = align256(sizeof(NoAllocDesc));
+  const size_t need = desc_bytes + mid_board_bytes(I);
+  if (need > kScratchBytes) {
+    set_error("board with " + std::to_string(I.spots) + " empty cells needs " + std::to_string(need) +
+              " bytes of device scratch; the allocation-free solve has " + std::to_string(kScratchBytes) +
+              " (boards with at most 15 empty cells, i.e. at least 21 stones)");
+    return PE_ERR_CAPACITY;
+  }
+  int dev = 0;
+  PE_HIP(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kScratchDevices) { set_error("device index out of range"); return PE_ERR_DEVICE; }
+  std::lock_guard<std::mutex> lock(g_scratch_mu[dev]);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    PE_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = static_cast<uint8_t*>(p);
+  }
+  uint8_t* const base = g_scratch_ptr[dev];
+  // every table, workspace and result entry the kernels read is written by them first
+  // (prep, then level n = spots .. 0), so nothing in the array needs clearing between calls
+  NoAllocDesc d;
+  std::memset(&d, 0, sizeof d);
+  d.board.I = I;
+  mid_carve(&d.board, base + desc_bytes);
+  const MidSizes sz = mid_sizes(I);
+  const int steps = I.spots + 1;
+  d.prefix[1] = sz.set0 + sz.wins1 + sz.set1p;
+  for (int t = 0; t < steps; t++) d.prefix[2 * (t + 1) + 1] = I.positions[I.spots - t];
+  PE_HIP(hipMemcpyAsync(base, &d, sizeof d, hipMemcpyHostToDevice, stream), "hipMemcpyAsync(descriptors)");
+  const auto* dboard = reinterpret_cast<const MidBoardDev*>(base);
+  const auto* dprefix = reinterpret_cast<const int64_t*>(base + offsetof(NoAllocDesc, prefix));
+  if (d.prefix[1] > 0) {
+    hipLaunchKernelGGL(mid_prep_kernel, dim3(unsigned((d.prefix[1] + 255) / 256)), dim3(256), 0, stream, dboard,
+                       dprefix, 1, d.prefix[1], T);
+    PE_HIP(hipGetLastError(), "mid_prep_kernel launch");
+  }
+  for (int t = 0; t < steps; t++) {
+    const int64_t total = d.prefix[2 * (t + 1) + 1];
+    if (total <= 0) continue;
+    const LevelGrid g = level_grid(I, I.spots - t);
+    hipLaunchKernelGGL(mid_level_kernel, dim3(g.blocks), dim3(256), 0, stream, dboard, dprefix + 2 * (t + 1), 1, total,
+                       t, g, T);
+    PE_HIP(hipGetLastError(), "mid_level_kernel launch");
+  }
+  PE_HIP(hipMemcpyAsync(results, d.board.results, sizeof(pe_halfsupers) * 19, hipMemcpyDeviceToHost, stream),
+         "hipMemcpyAsync(results)");
+  PE_HIP(hipStreamSynchronize(stream), "allocation-free midsolve");
+  for (int i = I.spots + 1; i < 19; i++) std::memset(&results[i], 0, sizeof(pe_halfsupers));
   return PE_OK;
 }
 
