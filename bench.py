@@ -530,8 +530,27 @@ def mid_bench(P, hip, n_boards=4096, seed=2):
         "batched_over_plain": plain_s / batched_s,
         "mismatches": mism, "plan": info,
         "algorithmic_GBps": work / batched_s / 1e9,
+        # HBM bound of the mix: its algorithmic bytes (32 B per position read or written, both
+        # passes in one) at 8 TB/s; batching cannot beat plain launches by more than
+        # plain time / this floor unless the kernels read less than the algorithmic bytes
+        "roofline": {"bound": "hbm", "achieved": work / batched_s / 1e9, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": work / batched_s / 1e9 / HBM_PEAK_GBPS, "traffic": mid_traffic(),
+                     "kernel": "batched plan (mid_level1_kernel, mid_boards_kernel, mid_prep_kernel)",
+                     "bytes_per_run": work, "run_ms": batched_s * 1e3},
+        "hbm_floor_s": work / (HBM_PEAK_GBPS * 1e9),
+        "max_batched_over_plain_at_hbm_peak": plain_s / (work / (HBM_PEAK_GBPS * 1e9)),
     })
     return out
+
+
+def mid_traffic():
+    """HBM bytes per run of the batched midsolve workload from the committed counter profile
+    (FETCH_SIZE and WRITE_SIZE scaled by the copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "mid_batched_4096boards_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_run")
 
 
 def serve_bench(P, rates=(200, 1000, 5000), seed=3, seconds=2.0):
@@ -675,6 +694,15 @@ def main():
             sum(b.nbytes for *_, b in outs) / 1e9, len(kids), sum(b.nbytes for *_, b in kids) / 1e9))
 
         stream = hip.Stream()
+        launch = None
+        if args.mode == 0:
+            # launch form: static grid against the persistent queue-fed grid, fastest kept
+            t0 = time.time()
+            forms, choice = plan.autotune(stream.ptr)
+            launch = {"choice": "grid" if choice == 0 else "queue, %d workgroups per CU" % choice,
+                      "timed_ms": {("grid" if f == 0 else "queue_%d_per_cu" % f): ms for f, ms in forms.items()},
+                      "seconds": time.time() - t0}
+            log("launch forms: %s" % launch)
         for _ in range(args.warmup):
             plan.run(stream.ptr)
         stream.synchronize()
@@ -726,6 +754,7 @@ def main():
                        "boxes": [{"section": "".join("%d%d" % c for c in sec), "lo": lo, "hi": hi} for sec, lo, hi in specs],
                        "output_nodes": n_out, "child_nodes_read": child_nodes,
                        "mode": "fused" if args.mode == 0 else "line passes"},
+            "launch": launch,
             "hbm_GBps_algorithmic": alg_step / (ms_per_step / 1e3) / 1e9,
             "step_ms": step_ms,
             "wall_s": wall,

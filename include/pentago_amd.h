@@ -123,6 +123,27 @@ pe_status pe_end_plan_run(const pe_end_plan* plan, void* stream);
 /* Output nodes one run computes. */
 int64_t pe_end_plan_nodes(const pe_end_plan* plan);
 pe_status pe_end_plan_destroy(pe_end_plan* plan);
+/* Launch form of a plan's fused blocks (the blocks of every section of the plan, of any
+ * size, in one launch).  0: a static grid, 16 workgroups per block.  n > 0: a persistent
+ * grid of n workgroups per CU fed from device work queues, one per XCD over that XCD's run
+ * of blocks, workgroups taking from the other XCDs' queues once their own is empty.  The
+ * batched form of the reference's per-microline job pool (pentago/end/compute.cc:378-404).
+ * Runs of one plan must not overlap (they share the queue heads). */
+pe_status pe_end_plan_set_launch(pe_end_plan* plan, int queue_wgs_per_cu);
+/* Time the launch forms {grid, 2, 3, 4 workgroups per CU} on this plan (warm-up, then the
+ * faster of two runs each, on `stream`; the plan's output is recomputed, identical each
+ * time), keep the fastest for the plan and as the current device's default for plans created
+ * later.  No counterpart in the reference. */
+typedef struct {
+  int32_t queue_wgs_per_cu;   /* the choice (0: grid) */
+  int32_t candidates;
+  int32_t forms[8];
+  float ms[8];                /* timed milliseconds per candidate */
+} pe_end_tuning;
+pe_status pe_end_plan_autotune(pe_end_plan* plan, void* stream, pe_end_tuning* out);
+/* The launch form the current device's later plans start with (what pe_end_plan_autotune
+ * sets). */
+pe_status pe_end_set_default_launch(int queue_wgs_per_cu);
 
 /* Section slabs: the device-resident storage unit.  A dense box of blocks [lo, hi) per axis
  * of one section, block (b - lo) in C-order over (hi - lo), one 4096-node slot per block,
@@ -310,7 +331,8 @@ typedef struct {
   int32_t workgroups;         /* persistent workgroups for small boards (0: a full device) */
   int32_t threads;            /* threads per small-board workgroup: 256, 512, 1024 (0: 512) */
   int32_t streams;            /* streams for large boards (0: 2; negative: all on the run stream) */
-  int32_t use_graph;          /* nonzero: pe_mid_plan_run replays a captured hipGraph */
+  int32_t use_graph;          /* > 0: pe_mid_plan_run replays a captured hipGraph; < 0: eager
+                                 launches; 0: the device default (pe_mid_autotune's choice) */
   int32_t pad;
 } pe_mid_config;
 typedef struct {
@@ -330,8 +352,9 @@ pe_status pe_mid_plan_destroy(pe_mid_plan* plan);
  * configuration would use, or PE_ERR_CAPACITY.  No device calls. */
 pe_status pe_mid_schedule_info(int n, const pe_high_board* boards, const pe_mid_config* config, int64_t* launches,
                                int64_t* workspace_bytes);
-/* Time (small_spots, threads) pairs on these boards and keep the fastest as the current
- * device's default.  The chosen configuration is returned through *best. */
+/* Time (small_spots, threads) pairs on these boards, then the fastest of them replayed as a
+ * captured hipGraph against eager launches, and keep the fastest as the current device's
+ * default.  The chosen configuration is returned through *best (use_graph 1 or -1). */
 pe_status pe_mid_autotune(int n, const pe_high_board* boards, int64_t workspace_budget, pe_mid_config* best);
 
 /* Serving loop: single-position solve requests admitted as they arrive.  A worker thread

@@ -97,7 +97,7 @@ class MidConfig(C.Structure):
     def as_dict(self):
         return {"workspace_budget": int(self.workspace_budget), "small_spots": int(self.small_spots),
                 "workgroups": int(self.workgroups), "threads": int(self.threads), "streams": int(self.streams),
-                "use_graph": bool(self.use_graph)}
+                "use_graph": self.use_graph > 0}
 
 
 class MidPlanStats(C.Structure):
@@ -117,6 +117,11 @@ class MidServerInfo(C.Structure):
 
 class MidValue(C.Structure):
     _fields_ = [("raw", C.c_uint64), ("value", C.c_int32), ("pad", C.c_int32)]
+
+
+class EndTuning(C.Structure):
+    _fields_ = [("queue_wgs_per_cu", C.c_int32), ("candidates", C.c_int32), ("forms", C.c_int32 * 8),
+                ("ms", C.c_float * 8)]
 
 
 WAKEUP = C.CFUNCTYPE(None, C.c_void_p)
@@ -150,6 +155,9 @@ def lib():
             "pe_end_slabs_plan_create": (C.c_int, [C.c_int, P(Slab), C.c_int, P(Slab), C.c_int, P(V)]),
             "pe_end_plan_nodes": (C.c_int64, [V]),
             "pe_end_plan_destroy": (C.c_int, [V]),
+            "pe_end_plan_set_launch": (C.c_int, [V, C.c_int]),
+            "pe_end_set_default_launch": (C.c_int, [C.c_int]),
+            "pe_end_plan_autotune": (C.c_int, [V, V, P(EndTuning)]),
             "pe_fill_seeded": (C.c_int, [V, C.c_int64, C.c_uint64, C.c_int64, V]),
             "pe_end_count_blocks": (C.c_int, [P(Slab), V, V]),
             "pe_end_count_slabs": (C.c_int, [C.c_int, P(Slab), V]),
@@ -281,6 +289,11 @@ def schedule_compute_line(line, d_input, d_output, wakeup, stream=None):
     return cb
 
 
+def set_default_launch(queue_wgs_per_cu):
+    """pe_end_set_default_launch: launch form of the current device's later end plans."""
+    _check(lib().pe_end_set_default_launch(int(queue_wgs_per_cu)), "pe_end_set_default_launch")
+
+
 class Plan:
     """A device-resident set of line jobs (pe_end_plan), launched with run()."""
 
@@ -325,6 +338,16 @@ class Plan:
 
     def run(self, stream=None):
         _check(lib().pe_end_plan_run(self.handle, stream), "pe_end_plan_run")
+
+    def set_launch(self, queue_wgs_per_cu):
+        """0: static grid; n > 0: persistent grid of n workgroups per CU on device queues."""
+        _check(lib().pe_end_plan_set_launch(self.handle, int(queue_wgs_per_cu)), "pe_end_plan_set_launch")
+
+    def autotune(self, stream=None):
+        """Time the launch forms on this plan and keep the fastest: {form: ms}, choice."""
+        t = EndTuning()
+        _check(lib().pe_end_plan_autotune(self.handle, stream, C.byref(t)), "pe_end_plan_autotune")
+        return {int(t.forms[i]): float(t.ms[i]) for i in range(t.candidates)}, int(t.queue_wgs_per_cu)
 
     def close(self):
         if self.handle:
@@ -570,11 +593,12 @@ class MidPlan:
     (n, 19, 4) uint64 array (win lo, win hi, notlose lo, notlose hi per entry)."""
 
     def __init__(self, boards, workspace_budget=0, small_spots=0, workgroups=0, threads=0, streams=0,
-                 use_graph=False):
+                 use_graph=None):
         self.boards = list(boards)
         n = len(self.boards)
         arr = (HighBoard * max(n, 1))(*self.boards)
-        cfg = MidConfig(workspace_budget, small_spots, workgroups, threads, streams, int(bool(use_graph)), 0)
+        graph = 0 if use_graph is None else (1 if use_graph else -1)   # None: the device default
+        cfg = MidConfig(workspace_budget, small_spots, workgroups, threads, streams, graph, 0)
         self.ptr = C.c_void_p()
         _check(lib().pe_mid_plan_create(n, arr, C.byref(cfg), C.byref(self.ptr)), "pe_mid_plan_create")
 

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -38,6 +39,8 @@ std::mutex g_dev_mutex;
 constexpr int kMaxDevices = 64;
 DevTables g_dev[kMaxDevices];
 bool g_dev_ready[kMaxDevices];
+// launch form of new end-engine plans per device (pe_end_plan_autotune's choice; 0: grid)
+std::atomic<int> g_end_form[kMaxDevices];
 }  // namespace
 
 void set_error(const std::string& e) { g_error = e; }
@@ -230,6 +233,14 @@ struct pe_end_plan {
   int64_t nodes = 0;
 };
 
+namespace {
+void apply_default_form(pe_end_plan* plan) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return;
+  plan->fused.queue_wgs_per_cu = plan->fused35.queue_wgs_per_cu = g_end_form[dev].load();
+}
+}  // namespace
+
 pe_status pe_end_plan_create(int n, const pe_line_job* jobs, pe_end_plan** out) {
   PE_GUARD_BEGIN
   if (n < 0 || !out || (n && !jobs)) return fail(PE_ERR_INVALID, "bad plan arguments");
@@ -287,6 +298,7 @@ pe_status pe_end_plan_create(int n, const pe_line_job* jobs, pe_end_plan** out) 
     delete plan;
     return fail(s, err);
   }
+  apply_default_form(plan);
   *out = plan;
   return PE_OK;
   PE_GUARD_END
@@ -345,6 +357,7 @@ pe_status pe_end_slabs_plan_create(int noutputs, const pe_slab* outputs, int nch
     delete plan;
     return fail(s, err.empty() ? pe_last_error() : err);
   }
+  apply_default_form(plan);
   *out = plan;
   return PE_OK;
   PE_GUARD_END
@@ -368,6 +381,70 @@ pe_status pe_end_plan_run(const pe_end_plan* plan, void* stream) {
 }
 
 int64_t pe_end_plan_nodes(const pe_end_plan* plan) { return plan ? plan->nodes : -1; }
+
+pe_status pe_end_plan_set_launch(pe_end_plan* plan, int queue_wgs_per_cu) {
+  PE_GUARD_BEGIN
+  if (!plan || queue_wgs_per_cu < 0 || queue_wgs_per_cu > 16) return fail(PE_ERR_INVALID, "bad launch form");
+  plan->fused.queue_wgs_per_cu = plan->fused35.queue_wgs_per_cu = queue_wgs_per_cu;
+  return PE_OK;
+  PE_GUARD_END
+}
+
+pe_status pe_end_set_default_launch(int queue_wgs_per_cu) {
+  PE_GUARD_BEGIN
+  int dev = 0;
+  if (queue_wgs_per_cu < 0 || queue_wgs_per_cu > 16) return fail(PE_ERR_INVALID, "bad launch form");
+  PE_HIP(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return fail(PE_ERR_DEVICE, "device index out of range");
+  g_end_form[dev] = queue_wgs_per_cu;
+  return PE_OK;
+  PE_GUARD_END
+}
+
+pe_status pe_end_plan_autotune(pe_end_plan* plan, void* stream, pe_end_tuning* out) {
+  PE_GUARD_BEGIN
+  if (!plan) return fail(PE_ERR_INVALID, "null plan");
+  const hipStream_t s = static_cast<hipStream_t>(stream);
+  const int32_t forms[] = {0, 2, 3, 4};
+  constexpr int kForms = 4;
+  pe_end_tuning t;
+  std::memset(&t, 0, sizeof t);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  PE_HIP(hipEventCreate(&e0), "hipEventCreate");
+  if (hipEventCreate(&e1) != hipSuccess) { (void)hipEventDestroy(e0); return fail(PE_ERR_DEVICE, "hipEventCreate"); }
+  pe_status st = PE_OK;
+  float best = -1;
+  for (int c = 0; c < kForms && st == PE_OK; c++) {
+    plan->fused.queue_wgs_per_cu = plan->fused35.queue_wgs_per_cu = forms[c];
+    st = pe_end_plan_run(plan, stream);   // warm-up
+    float ms = -1;
+    for (int rep = 0; rep < 2 && st == PE_OK; rep++) {   // the faster of two timed runs
+      float x = 0;
+      if (hipEventRecord(e0, s) != hipSuccess) st = fail(PE_ERR_DEVICE, "hipEventRecord");
+      if (st == PE_OK) st = pe_end_plan_run(plan, stream);
+      if (st == PE_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                          hipEventElapsedTime(&x, e0, e1) != hipSuccess))
+        st = fail(PE_ERR_DEVICE, "autotune timing failed");
+      if (st == PE_OK && (ms < 0 || x < ms)) ms = x;
+    }
+    t.forms[c] = forms[c];
+    t.ms[c] = ms;
+    if (st == PE_OK && (best < 0 || ms < best)) {
+      best = ms;
+      t.queue_wgs_per_cu = forms[c];
+    }
+  }
+  t.candidates = kForms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  plan->fused.queue_wgs_per_cu = plan->fused35.queue_wgs_per_cu = t.queue_wgs_per_cu;
+  if (st != PE_OK) return st;
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess && dev >= 0 && dev < kMaxDevices) g_end_form[dev] = t.queue_wgs_per_cu;
+  if (out) *out = t;
+  return PE_OK;
+  PE_GUARD_END
+}
 
 pe_status pe_end_plan_destroy(pe_end_plan* plan) {
   PE_GUARD_BEGIN
@@ -452,7 +529,7 @@ int64_t pe_midsolve_workspace_size(int min_slice) {
 
 namespace {
 
-struct TunedMid { bool set = false; int32_t small_spots = 0, threads = 0, workgroups = 0; };
+struct TunedMid { bool set = false; int32_t small_spots = 0, threads = 0, workgroups = 0, use_graph = 0; };
 TunedMid g_tuned[kMaxDevices];
 
 pe_status mid_infos(int n, const pe_high_board* boards, std::vector<MidInfo>* infos) {
@@ -488,7 +565,7 @@ pe_status mid_resolve_config(const pe_mid_config* in, MidConfig* out) {
   out->workgroups = c.workgroups;
   out->threads = c.threads;
   out->streams = c.streams;
-  out->graph = c.use_graph != 0;
+  out->graph = c.use_graph > 0 || (c.use_graph == 0 && t.set && t.use_graph);
   return PE_OK;
 }
 
@@ -658,6 +735,30 @@ pe_status pe_mid_autotune(int n, const pe_high_board* boards, int64_t workspace_
         chosen.streams = cfg.streams;
       }
     }
+  // the chosen configuration replayed as a captured graph (launch overhead of the large
+  // boards' level launches) against eager launches
+  chosen.use_graph = -1;
+  if (st == PE_OK) {
+    pe_mid_config c = chosen;
+    c.use_graph = 1;
+    MidConfig cfg;
+    st = mid_resolve_config(&c, &cfg);
+    MidPlan plan;
+    if (st == PE_OK) st = mid_plan_create(infos, cfg, &plan);
+    float ms = -1;
+    if (st == PE_OK) st = mid_plan_run(plan, s);   // warm-up (captures the graph)
+    for (int rep = 0; rep < 2 && st == PE_OK; rep++) {
+      float t = 0;
+      if (hipEventRecord(e0, s) == hipSuccess) st = mid_plan_run(plan, s);
+      if (st == PE_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                          hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+        st = fail(PE_ERR_DEVICE, "autotune timing failed");
+      if (st == PE_OK && (ms < 0 || t < ms)) ms = t;
+    }
+    (void)hipStreamSynchronize(s);
+    mid_plan_destroy(&plan);
+    if (st == PE_OK && ms < best_ms) chosen.use_graph = 1;
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipStreamDestroy(s);
@@ -666,6 +767,7 @@ pe_status pe_mid_autotune(int n, const pe_high_board* boards, int64_t workspace_
   g_tuned[dev].small_spots = chosen.small_spots;
   g_tuned[dev].threads = chosen.threads;
   g_tuned[dev].workgroups = chosen.workgroups;
+  g_tuned[dev].use_graph = chosen.use_graph > 0;
   *best = chosen;
   return PE_OK;
   PE_GUARD_END

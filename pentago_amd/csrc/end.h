@@ -108,7 +108,8 @@ struct FusedPlan {
   void* blob = nullptr;
   int nblocks = 0;
   bool s35 = false;   // blocks of 35-stone sections (the kernel's no-children form)
-  size_t off_lines = 0, off_secs = 0, off_moves = 0, off_refl = 0;
+  size_t off_lines = 0, off_secs = 0, off_moves = 0, off_refl = 0, off_heads = 0;
+  int queue_wgs_per_cu = 0;   // 0: one workgroup per (block, part); else persistent queue form
   int64_t output_nodes = 0;
 };
 

@@ -253,6 +253,40 @@ void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __r
   fused_block<S35>(B, secs[B.sec], lines, fields, movetab, refl, part);
 }
 
+// The same blocks fed from device work queues to a persistent grid: heads[x] counts the
+// tasks (block * 16 + part, from the start of XCD x's run of blocks) taken from XCD x's
+// run.  A workgroup takes tasks from the run of its own XCD (workgroup i runs on XCD
+// i % 8) and, once that is exhausted, from the other runs in turn; every pop past a run's
+// end fails, so each workgroup stops after at most eight failed pops.
+template <bool S35>
+__global__ __launch_bounds__(kThreads)
+void fused_blocks_queue_kernel(const BlockDev* __restrict__ blocks, const LineDev* __restrict__ lines,
+                               const SecDev* __restrict__ secs, const uint64_t* __restrict__ fields,
+                               const uint16_t* __restrict__ movetab, const uint8_t* __restrict__ refl,
+                               int nblocks, int blocks_per_xcd, unsigned int* __restrict__ heads) {
+  __shared__ int s_task;
+  const int home = int(blockIdx.x) & 7;
+  for (int x = 0; x < 8;) {
+    const int xcd = (home + x) & 7;
+    const int run = max(0, min(blocks_per_xcd, nblocks - xcd * blocks_per_xcd));
+    if (threadIdx.x == 0) s_task = int(atomicAdd(&heads[xcd], 1u));
+    __syncthreads();
+    const int t = s_task;
+    __syncthreads();   // every thread has read the task before the next pop overwrites it
+    if (t >= run * kParts) {
+      x++;
+      continue;
+    }
+    const BlockDev& B = blocks[xcd * blocks_per_xcd + t / kParts];
+    fused_block<S35>(B, secs[B.sec], lines, fields, movetab, refl, t % kParts);
+    __syncthreads();   // the block's staged fields are no longer read
+  }
+}
+
+__global__ void fused_queue_reset_kernel(unsigned int* heads) {
+  if (threadIdx.x < 8) heads[threadIdx.x] = 0;
+}
+
 // One block line passed by value (no device descriptors): the line and its section record.
 struct LineLaunch {
   LineDev line;
@@ -390,7 +424,8 @@ static pe_status pack_fused_plan(const std::vector<BlockDev>& blocks, const std:
   const size_t o_moves = o_secs + up(sizeof(SecDev) * secs.size());
   const size_t o_offs = o_moves + up(sizeof(uint16_t) * std::max<size_t>(movetab.size(), 1));
   const size_t o_refl = o_offs + up(sizeof(int64_t) * offs.size());
-  const size_t bytes = o_refl + refl.size();
+  const size_t o_heads = o_refl + up(refl.size());
+  const size_t bytes = o_heads + 256;
   void* d = nullptr;
   if (hipMalloc(&d, bytes) != hipSuccess) { *err = "hipMalloc(fused plan) failed"; return PE_ERR_DEVICE; }
   auto* base = static_cast<uint8_t*>(d);
@@ -420,6 +455,7 @@ static pe_status pack_fused_plan(const std::vector<BlockDev>& blocks, const std:
   plan->off_secs = o_secs;
   plan->off_moves = o_moves;
   plan->off_refl = o_refl;
+  plan->off_heads = o_heads;
   plan->output_nodes = nodes;
   return PE_OK;
 }
@@ -635,14 +671,31 @@ pe_status fused_plan_run(const FusedPlan& plan, hipStream_t stream) {
   if (device_tables(&T, &err) != PE_OK) { set_error(err); return PE_ERR_DEVICE; }
   auto* base = static_cast<uint8_t*>(plan.blob);
   const int per_xcd = (plan.nblocks + 7) / 8;
+  const auto* blocks = reinterpret_cast<const BlockDev*>(base);
+  const auto* lines = reinterpret_cast<const LineDev*>(base + plan.off_lines);
+  const auto* secs = reinterpret_cast<const SecDev*>(base + plan.off_secs);
+  const auto* moves = reinterpret_cast<const uint16_t*>(base + plan.off_moves);
+  const uint8_t* refl = base + plan.off_refl;
+  if (plan.queue_wgs_per_cu > 0) {
+    // persistent form: a whole number of workgroups per XCD, queue heads cleared by a kernel
+    // on the same stream (no memset node, so the sequence may be captured in a graph)
+    int dev = 0, cus = 0;
+    PE_HIP(hipGetDevice(&dev), "hipGetDevice");
+    PE_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute(CUs)");
+    const int64_t want = int64_t(cus) * plan.queue_wgs_per_cu;
+    const unsigned grid = unsigned(std::max<int64_t>(8, std::min<int64_t>(want, int64_t(per_xcd) * kParts * 8)) / 8 * 8);
+    auto* heads = reinterpret_cast<unsigned int*>(base + plan.off_heads);
+    hipLaunchKernelGGL(fused_queue_reset_kernel, dim3(1), dim3(64), 0, stream, heads);
+    hipLaunchKernelGGL(plan.s35 ? fused_blocks_queue_kernel<true> : fused_blocks_queue_kernel<false>, dim3(grid),
+                       dim3(kThreads), 0, stream, blocks, lines, secs, T.fields, moves, refl, plan.nblocks, per_xcd,
+                       heads);
+    PE_HIP(hipGetLastError(), "fused_blocks_queue_kernel launch");
+    return PE_OK;
+  }
   const int64_t grid = int64_t(per_xcd) * kParts * 8;
   if (grid > 0x7fffffff) { set_error("fused plan too large"); return PE_ERR_CAPACITY; }
   hipLaunchKernelGGL(plan.s35 ? fused_blocks_kernel<true> : fused_blocks_kernel<false>, dim3(unsigned(grid)),
-                     dim3(kThreads), 0, stream,
-                     reinterpret_cast<const BlockDev*>(base), reinterpret_cast<const LineDev*>(base + plan.off_lines),
-                     reinterpret_cast<const SecDev*>(base + plan.off_secs), T.fields,
-                     reinterpret_cast<const uint16_t*>(base + plan.off_moves), base + plan.off_refl, plan.nblocks,
-                     per_xcd);
+                     dim3(kThreads), 0, stream, blocks, lines, secs, T.fields, moves, refl, plan.nblocks, per_xcd);
   PE_HIP(hipGetLastError(), "fused_blocks_kernel launch");
   return PE_OK;
 }

@@ -58,7 +58,8 @@ struct Batch {
 struct MidServer {
   int device = 0;
   int32_t max_batch = 512;
-  int64_t budget = 0, in_use = 0, peak = 0;
+  int64_t budget = 0, in_use = 0;
+  int64_t footprint_peak = 0;            // largest device bytes held by the arenas together
   MidConfig cfg{};
   std::vector<hipStream_t> streams;
   std::vector<hipEvent_t> events;
@@ -76,7 +77,41 @@ struct MidServer {
   std::deque<int64_t> pending;
   int64_t next_ticket = 1, submitted = 0, completed = 0, batches = 0;
   bool stop = false;
+  bool broken = false;   // the worker stopped on an error: no request is admitted any more
   std::thread worker;
+
+  // Mark requests that never reached a stream as failed.  Caller holds no lock.
+  void fail_tickets(const std::vector<int64_t>& tickets, const std::string& e) {
+    std::lock_guard<std::mutex> lock(mu);
+    const auto now = Clock::now();
+    for (int64_t t : tickets) {
+      const auto it = requests.find(t);
+      if (it == requests.end()) continue;
+      it->second.status = PE_ERR_CAPACITY;
+      it->second.error = e;
+      it->second.done = true;
+      it->second.done_at = now;
+      completed++;
+    }
+    cv_done.notify_all();
+  }
+
+  // Every request not finished yet fails (the worker could not continue).
+  void fail_all(const std::string& e) {
+    std::lock_guard<std::mutex> lock(mu);
+    const auto now = Clock::now();
+    for (auto& kv : requests)
+      if (!kv.second.done) {
+        kv.second.status = PE_ERR_CAPACITY;
+        kv.second.error = e;
+        kv.second.done = true;
+        kv.second.done_at = now;
+        completed++;
+      }
+    pending.clear();
+    stop = broken = true;
+    cv_done.notify_all();
+  }
 
   // Finish the requests of a batch (results from staging when ok).  Caller holds no lock.
   void finish(int k, pe_status st, const std::string& e) {
@@ -105,7 +140,22 @@ struct MidServer {
     stream_busy[size_t(k)] = 0;
   }
 
-  // Launch the batch on stream k: plan (stream-ordered memory), run, results to staging.
+  // Device bytes held by the arenas of the busy streams other than k, and by every arena
+  // other than k (k = -1: all).  Only the worker thread changes arenas and busy flags.
+  int64_t held_busy_except(int k) const {
+    int64_t h = 0;
+    for (size_t j = 0; j < arena.size(); j++)
+      if (int(j) != k && stream_busy[j]) h += int64_t(arena_cap[j]);
+    return h;
+  }
+  size_t held_all_except(int k) const {
+    size_t h = 0;
+    for (size_t j = 0; j < arena.size(); j++)
+      if (int(j) != k) h += arena_cap[j];
+    return h;
+  }
+
+  // Launch the batch on stream k: plan (in the stream's arena), run, results to staging.
   pe_status launch(int k) {
     Batch& b = batch_of_stream[size_t(k)];
     const hipStream_t s = streams[size_t(k)];
@@ -131,30 +181,28 @@ struct MidServer {
     }
     // each stream keeps one device arena (batches on a stream run one after another), so a
     // batch allocates nothing once the arena has grown to the largest batch it runs;
-    // per-batch stream-ordered allocations made concurrent streams wait on each other
-    // The arenas together stay within the budget: idle streams give theirs up when a
-    // stream needs to grow, and a batch that still does not fit falls back to a
-    // stream-ordered allocation of its own.
+    // per-batch stream-ordered allocations made concurrent streams wait on each other.
+    // Admission keeps every arena of a busy stream plus this one within the budget, so the
+    // arena can always grow here once idle streams give theirs up.
     if (L.bytes > arena_cap[size_t(k)]) {
-      const size_t cap = std::max<size_t>(L.bytes + L.bytes / 4, size_t(64) << 20);
-      auto held = [&] {
-        size_t h = 0;
-        for (size_t j = 0; j < arena.size(); j++) h += j == size_t(k) ? 0 : arena_cap[j];
-        return h;
-      };
-      for (size_t j = 0; j < arena.size() && held() + cap > size_t(budget); j++)
-        if (j != size_t(k) && !stream_busy[j] && arena[j]) {
+      const size_t room = size_t(budget) - size_t(held_busy_except(k));
+      if (L.bytes > room) {
+        set_error("batch exceeds the workspace budget left by the batches in flight");
+        return PE_ERR_CAPACITY;
+      }
+      const size_t cap = std::min(room, std::max<size_t>(L.bytes + L.bytes / 4, size_t(64) << 20));
+      for (size_t j = 0; j < arena.size(); j++)
+        if (j != size_t(k) && !stream_busy[j] && arena[j] && held_all_except(k) + cap > size_t(budget)) {
           (void)hipFree(arena[j]);
           arena[j] = nullptr;
           arena_cap[j] = 0;
         }
-      if (held() + cap <= size_t(budget)) {
-        if (arena[size_t(k)]) (void)hipFree(arena[size_t(k)]);
-        arena[size_t(k)] = nullptr;
-        arena_cap[size_t(k)] = 0;
-        PE_HIP(hipMalloc(&arena[size_t(k)], cap), "hipMalloc(server arena)");
-        arena_cap[size_t(k)] = cap;
-      }
+      if (arena[size_t(k)]) (void)hipFree(arena[size_t(k)]);
+      arena[size_t(k)] = nullptr;
+      arena_cap[size_t(k)] = 0;
+      PE_HIP(hipMalloc(&arena[size_t(k)], cap), "hipMalloc(server arena)");
+      arena_cap[size_t(k)] = cap;
+      footprint_peak = std::max(footprint_peak, int64_t(held_all_except(-1)));
     }
     pe_status st = mid_plan_create(infos, cfg, &b.plan, s, upload[size_t(k)], upload_cap[size_t(k)], arena[size_t(k)],
                                    arena_cap[size_t(k)]);
@@ -192,7 +240,7 @@ struct MidServer {
         if (stream_busy[k]) continue;
         Batch b;
         std::vector<MidInfo> infos;
-        {
+        try {
           std::lock_guard<std::mutex> lock(mu);
           const auto now = Clock::now();
           // A large board (more empty cells than small_spots: full-device level launches,
@@ -226,7 +274,10 @@ struct MidServer {
             const int64_t per_board = int64_t(sizeof(MidInfo) + 8 * sizeof(void*) + 3 * sizeof(int64_t) +
                                               19 * sizeof(pe_halfsupers));
             const int64_t bound = t2 + n2 * w2 + s2 + int64_t(infos.size()) * per_board + 5 * 256 + 4096;
-            if (in_use + bound > budget && in_use + batch_bytes(infos) > budget) {
+            // the device footprint once this batch runs: the arenas of the busy streams and
+            // this stream's arena grown to the batch (idle streams give theirs up)
+            const int64_t others = held_busy_except(int(k)), own = int64_t(arena_cap[k]);
+            if (others + std::max(own, bound) > budget && others + std::max(own, batch_bytes(infos)) > budget) {
               infos.pop_back();
               // the oldest waiting request blocks further admission once it has waited long
               if (it == pending.begin() && now - r.submitted > kHeadOfLine) break;
@@ -244,17 +295,34 @@ struct MidServer {
           if (!b.tickets.empty()) b.bytes = batch_bytes(infos);
           if (!b.tickets.empty()) {
             in_use += b.bytes;
-            peak = std::max(peak, in_use);
             batches++;
           }
+        } catch (const std::exception& e) {   // host memory exhausted while forming the batch
+          fail_tickets(b.tickets, std::string("serving loop: ") + e.what());
+          if (b.bytes) {
+            std::lock_guard<std::mutex> lock(mu);
+            in_use -= b.bytes;
+          }
+          progressed = true;
+          continue;
         }
         if (b.tickets.empty()) break;
         batch_of_stream[k] = std::move(b);
         stream_busy[k] = 1;
-        const pe_status st = launch(int(k));
+        pe_status st = PE_OK;
+        std::string why = "batch launch failed";
+        try {
+          st = launch(int(k));
+        } catch (const std::bad_alloc&) {
+          st = PE_ERR_CAPACITY;
+          why = "host memory exhausted while launching a batch";
+        } catch (const std::exception& e) {
+          st = PE_ERR_INVALID;
+          why = std::string("batch launch failed: ") + e.what();
+        }
         if (st != PE_OK) {
           (void)hipStreamSynchronize(streams[k]);
-          finish(int(k), st, "batch launch failed");
+          finish(int(k), st, why);
         }
         progressed = true;
       }
@@ -313,7 +381,13 @@ pe_status mid_server_create(const pe_mid_server_config* c, MidServer** out) {
   }
   S->stream_busy.assign(size_t(nstreams), 0);
   S->batch_of_stream.resize(size_t(nstreams));
-  S->worker = std::thread([S] { S->run(); });
+  S->worker = std::thread([S] {
+    try {
+      S->run();
+    } catch (const std::exception& e) {   // no exception leaves the worker: fail what is left
+      S->fail_all(std::string("serving loop stopped: ") + e.what());
+    }
+  });
   *out = S;
   return PE_OK;
 }
@@ -324,6 +398,7 @@ pe_status mid_server_submit(MidServer* S, const pe_high_board& board, int64_t* t
   if (S->batch_bytes({r.info}) > S->budget) { set_error("board needs more workspace than the server holds"); return PE_ERR_CAPACITY; }
   {
     std::lock_guard<std::mutex> lock(S->mu);
+    if (S->broken) { set_error("the serving loop stopped after an error"); return PE_ERR_DEVICE; }
     r.submitted = Clock::now();
     *ticket = S->next_ticket++;
     S->requests.emplace(*ticket, std::move(r));
@@ -372,7 +447,7 @@ void mid_server_stats(MidServer* S, pe_mid_server_info* out) {
   out->submitted = S->submitted;
   out->completed = S->completed;
   out->batches = S->batches;
-  out->peak_workspace = S->peak;
+  out->peak_workspace = S->footprint_peak;
   out->pending = int64_t(S->pending.size());
   out->in_flight = S->submitted - S->completed - int64_t(S->pending.size());
 }

@@ -174,6 +174,77 @@ def test_mixed_sections_in_one_plan(mode):
     plan.close()
 
 
+def download_outs(outs):
+    res = []
+    for counts, lo, hi, buf in outs:
+        nb = int(np.prod([h - l for l, h in zip(lo, hi)]))
+        res.append(buf.download(np.zeros(8 * 4096 * nb, dtype=np.uint64)))
+    return res
+
+
+def test_queue_launch_matches_grid_and_oracle():
+    """The persistent queue-fed form of the fused kernel (per-XCD queues, work taken from the
+    other XCDs' queues at the end) on boxes of two sections of uneven size: every workgroup
+    count gives the grid form's output bit for bit, and sampled blocks equal the oracle."""
+    P.init()
+    specs = [(bench.SECTION, [3] * 4, [6, 6, 5, 6]), (bench.SECTION2, [9, 9, 9, 9], [10, 11, 10, 11])]
+    outs, kids, plan, n_out, child_nodes = bench.build_boxes(P, hip, specs, 0)
+    plan.set_launch(0)
+    plan.run()
+    hip.synchronize()
+    grid = download_outs(outs)
+    for wgs in (1, 2, 3, 4):
+        for _, _, _, buf in outs:
+            buf.zero()
+        plan.set_launch(wgs)
+        print("queue form, %d workgroups per CU" % wgs, flush=True)
+        plan.run()
+        plan.run()   # a second run resets the queue heads
+        hip.synchronize()
+        for a, b in zip(grid, download_outs(outs)):
+            assert np.array_equal(a, b), wgs
+    S = SlabSet()
+    for child, lo, hi, buf in kids:
+        nb = int(np.prod([h - l for l, h in zip(lo, hi)]))
+        S.add(child, lo, hi, buf.download(np.zeros(8 * 4096 * nb, dtype=np.uint64)).reshape(nb, 4096, 8))
+    got = download_outs(outs)
+    for (counts, lo, hi, _), g in zip(outs, got):
+        nb = int(np.prod([h - l for l, h in zip(lo, hi)]))
+        g = g.reshape(nb, 4096, 8)
+        for idx in (0, nb - 1):
+            blk = np.unravel_index(idx, [h - l for l, h in zip(lo, hi)])
+            blk = tuple(int(lo[a] + blk[a]) for a in range(4))
+            expect = oracle_block(counts, blk, S.block)
+            assert np.array_equal(g[idx].reshape(-1)[:expect.size], expect), (counts, blk)
+    plan.close()
+
+
+def test_autotune_picks_a_form_and_keeps_the_output():
+    P.init()
+    specs = [(bench.SECTION, [3] * 4, [5] * 4), (bench.SECTION2, [9] * 4, [10] * 4)]
+    outs, kids, plan, n_out, child_nodes = bench.build_boxes(P, hip, specs, 0)
+    plan.set_launch(0)
+    plan.run()
+    hip.synchronize()
+    before = download_outs(outs)
+    times, choice = plan.autotune()
+    print("autotune", times, choice, flush=True)
+    assert set(times) == {0, 2, 3, 4} and choice in times
+    assert times[choice] == min(times.values())
+    for _, _, _, buf in outs:
+        buf.zero()
+    plan.run()
+    hip.synchronize()
+    for a, b in zip(before, download_outs(outs)):
+        assert np.array_equal(a, b)
+    plan.close()
+    # the choice became the device default of later plans; put the grid form back
+    plan2 = bench.build_boxes(P, hip, specs[:1], 0)[2]
+    assert P.lib().pe_end_plan_set_launch(plan2.handle, 0) == 0
+    P.set_default_launch(0)
+    plan2.close()
+
+
 class DeviceSlabs:
     """Child blocks read from device slabs on demand (for boxes too large to copy to the host)."""
 

@@ -216,23 +216,29 @@ def test_slice_32_matches_production_counts():
 
 
 def test_plan_mixing_slice_34_and_35_outputs():
-    """One slab plan with a 35-stone output (line pass, no children) and 34-stone outputs
-    (fused kernel) equals the two computed separately."""
+    """One slab plan with 35-stone outputs (the fused kernel's no-children form) and 34-stone
+    outputs (fused kernel) equals the two computed separately: the 35-stone sections through
+    line passes on the line kernel (mode 1), the 34-stone ones on a fused plan of their own.
+    Every valid node of every block (partial edge blocks included) is compared."""
     s35 = P.descendent_sections(EMPTY, 35)
     s34 = P.descendent_sections(EMPTY, 34)
     small34 = sorted(s34, key=lambda s: int(np.prod(P.section_shape(s))))[:4]
     kids = sorted({c for s in small34 for c in S.child_sections(s)})
     st35 = P.SliceStore(kids)
     st35.compute()
-    extra35 = [s for s in s35 if s not in set(kids)][:1]
+    extra35 = [s for s in s35 if s not in set(kids)][:3]
     a34, b34 = P.SliceStore(small34), P.SliceStore(small34)
     a35, b35 = P.SliceStore(extra35), P.SliceStore(extra35)
     children = [st35.slab(k) for k in range(len(kids))]
-    # separately
-    a35.compute()
+    # separately: the 35-stone sections by line passes (mode 1)
+    lined = P.Plan.slab([a35.slab(k) for k in range(len(extra35))], [], 1)
+    lined.run()
+    hip.synchronize()
+    lined.close()
     a34.compute(st35)
-    # together: one plan over both stores' slabs
-    plan = P.Plan.slab([b35.slab(0)] + [b34.slab(k) for k in range(len(small34))], children, 0)
+    # together: one fused plan over both stores' slabs
+    plan = P.Plan.slab([b35.slab(k) for k in range(len(extra35))] + [b34.slab(k) for k in range(len(small34))],
+                       children, 0)
     plan.run()
     hip.synchronize()
     plan.close()
