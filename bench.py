@@ -12,7 +12,7 @@ once (every node gets all its dimensions and is written once); mode 1 runs four 
 passes per box, OR-merging into the output.  Inputs are on the device before timing starts.
 
 Prints one JSON line.  Metric: superpositions/s = output nodes per second (the reference's
-output nodes/second, pentago/mpi/toplevel.cc:83-94).  The same line carries a box of the
+output nodes/second, pentago/mpi/toplevel.cc:83-94).  The same line carries the whole
 slice-30 section, whole slices 35, 34, 33 and 32 on real data (computed from nothing, counts
 checked against the production run's), the batched midsolve (boards/s against plain
 per-board launches) and serving (p50/p99 latency at Poisson loads) measurements unless
@@ -287,32 +287,140 @@ def line_kernel_bench(P, hip, steps, k=8):
     return out
 
 
-def section30_bench(P, hip, steps):
-    """The slice-30 section (4,4)(4,3)(3,4)(4,4) (standard form (3,4)(4,4)(4,4)(4,3), shape
-    318 x 160 x 160 x 318): a box of 10 x 20 x 20 x 10 blocks (all blocks along the two
-    middle axes; its dimension-2 children come through a reflection) in one fused launch."""
-    specs = [(SECTION30, [0, 0, 0, 0], [10, 20, 20, 10])]
-    outs, kids, plan, n_out, child_nodes = build_boxes(P, hip, specs, 0)
-    stream = hip.Stream()
-    plan.run(stream.ptr)
-    stream.synchronize()
+def slab_index(nb, b):
+    idx = 0
+    for a in range(4):
+        idx = idx * nb[a] + b[a]
+    return idx
+
+
+def section_by_passes(P, hip, counts, samples, stream, seed_base=0x5EC30000):
+    """A whole section computed one open dimension at a time: the output (every block, one
+    4096-node slot each) stays resident, and each pass holds only that dimension's child
+    section (seeded on the device, not timed; a child shared by two dimensions is filled
+    once).  A pass runs every line of its dimension as a whole-line job on the fused kernel
+    (pe_end_plan_create), the later passes OR-accumulating into the output (the block store's
+    merge, pentago/end/block_store.cc:173-229); each pass runs twice (OR-accumulating twice
+    leaves the same output) and the second run is timed.  For every sampled block the oracle's
+    line through it is computed from child data regenerated on the host from the seed.
+    Returns (output buffer, block counts, passes, {block: expected nodes})."""
+    from oracle import oracle as O
+    from tests.seeded import seeded_nodes
+    shape = P.section_shape(counts)
+    nb = [(s + 7) // 8 for s in shape]
+    n_out = int(np.prod(shape))
+    out = hip.Buffer(64 * 4096 * int(np.prod(nb)))
     ev0, ev1 = hip.Event(), hip.Event()
-    ms = []
-    for _ in range(max(steps, 1)):
-        ev0.record(stream)
-        plan.run(stream.ptr)
-        ev1.record(stream)
-        ev1.synchronize()
-        ms.append(ev0.elapsed_ms(ev1))
-    plan.close()
-    t = sum(ms) / len(ms)
-    alg = 64 * (n_out + child_nodes)
-    out = {"workload": "box [0,10)x[0,20)x[0,20)x[0,10) of section 30-%s" % "".join("%d%d" % c for c in SECTION30),
-           "output_nodes": n_out, "child_nodes_read": child_nodes, "ms": t,
-           "superpositions_per_s": n_out / (t / 1e3), "algorithmic_GBps": alg / (t / 1e3) / 1e9,
-           "roofline_frac": alg / (t / 1e3) / 1e9 / HBM_PEAK_GBPS}
-    log("section 30 box: %s" % out)
-    return out
+    expect = {b: None for b in samples}
+    passes = []
+    kid, kid_child = None, None
+    for d in range(4):
+        if sum(counts[d]) >= 9:
+            continue
+        info = P.line_describe(P.make_line(counts, d, tuple(0 for a in range(4) if a != d)))
+        child = info.standard_child.counts
+        cshape = P.section_shape(child)
+        cnb = [(s + 7) // 8 for s in cshape]
+        seed = seed_base + sum(c * 10 ** i for i, c in enumerate(sum(child, ())))
+        if child != kid_child:
+            kid = None
+            kid = hip.Buffer(64 * 4096 * int(np.prod(cnb)))
+            kid_child = child
+            P.fill_seeded(kid.ptr, 4096 * int(np.prod(cnb)), seed)
+        t0 = time.time()
+        jobs = []
+        others = [a for a in range(4) if a != d]
+        for base in np.ndindex(*[nb[a] for a in others]):
+            base = tuple(int(x) for x in base)
+            line = P.make_line(counts, d, base)
+            li = P.line_describe(line)
+            cd = li.child_dimension
+            in_offs = []
+            for k in range(li.input_blocks):
+                cb = list(li.first_child_block)
+                cb[cd] = k
+                in_offs.append(4096 * slab_index(cnb, cb))
+            out_offs = []
+            for k in range(li.length):
+                b = [0] * 4
+                for a, x in zip(others, base):
+                    b[a] = x
+                b[d] = k
+                out_offs.append(4096 * slab_index(nb, b))
+            jobs.append({"line": line, "input": kid.ptr, "output": out.ptr, "in_offsets": in_offs,
+                         "out_offsets": out_offs, "accumulate": bool(passes)})
+        plan = P.Plan(jobs)
+        build_s = time.time() - t0
+        hip.synchronize()
+        ms = []
+        for _ in range(2):
+            ev0.record(stream)
+            plan.run(stream.ptr)
+            ev1.record(stream)
+            ev1.synchronize()
+            ms.append(ev0.elapsed_ms(ev1))
+        plan.close()
+        passes.append({"dimension": d, "child": "".join("%d%d" % c for c in child), "lines": len(jobs),
+                       "child_GB": 64 * 4096 * int(np.prod(cnb)) / 1e9, "ms": ms[1], "first_run_ms": ms[0],
+                       "plan_build_s": build_s,
+                       "algorithmic_bytes": 64 * (int(np.prod(cshape)) + (2 if passes else 1) * n_out)})
+        log("section %s, dimension %d: %s" % ("".join("%d%d" % c for c in counts), d, passes[-1]))
+        for blk in samples:
+            base = tuple(blk[a] for a in range(4) if a != d)
+            ol = O.make_line(counts, d, base)
+            oi = O.line_describe(ol)
+            inp = np.zeros(8 * oi.input_nodes, dtype=np.uint64)
+            for kk in range(oi.input_blocks):
+                cb = list(oi.first_child_block)
+                cb[oi.child_dim] = kk
+                bn = int(np.prod([min(8, cshape[a] - 8 * cb[a]) for a in range(4)]))
+                off = 8 * kk * (oi.block_stride + 1)
+                inp[off:off + 8 * bn] = seeded_nodes(bn, seed, 4096 * slab_index(cnb, cb))
+            line_out = np.zeros(8 * oi.output_nodes, dtype=np.uint64)
+            O.compute_lines_threaded([ol], [inp], [line_out], baseline_threads())
+            bn = int(np.prod([min(8, shape[a] - 8 * blk[a]) for a in range(4)]))
+            part = line_out[8 * oi.block_stride * blk[d]:8 * oi.block_stride * blk[d] + 8 * bn]
+            expect[blk] = part if expect[blk] is None else expect[blk] | part
+    kid = None
+    return out, nb, passes, expect
+
+
+def section30_bench(P, hip, steps, check_blocks=2):
+    """The whole slice-30 section (4,4)(4,3)(3,4)(4,4) (standard form (3,4)(4,4)(4,4)(4,3),
+    shape 318 x 160 x 160 x 318, 2,588,774,400 nodes, 640,000 blocks).  The output stays
+    resident (167.8 GB of slab slots); the child sections (84 + 42 + 67 GB) do not fit beside
+    it together, so the section runs one open dimension at a time (section_by_passes).
+    Kernel time from HIP events, plan builds apart; sampled blocks against the oracle."""
+    counts = SECTION30
+    shape = P.section_shape(counts)
+    nb = [(s + 7) // 8 for s in shape]
+    n_out = int(np.prod(shape))
+    rng = np.random.default_rng(30)
+    samples = [tuple(int(rng.integers(n)) for n in nb) for _ in range(check_blocks - 1)] + [tuple(n - 1 for n in nb)]
+    stream = hip.Stream()
+    t_wall = time.time()
+    out, nb, passes, expect = section_by_passes(P, hip, counts, samples, stream)
+    wall = time.time() - t_wall
+    mism = 0
+    for blk, e in expect.items():
+        got = out.download(np.zeros(e.size, dtype=np.uint64), offset=64 * 4096 * slab_index(nb, blk))
+        mism += int(not np.array_equal(got, e))
+    del out
+    kernel_ms = sum(p["ms"] for p in passes)
+    alg = sum(p["algorithmic_bytes"] for p in passes)
+    res = {"workload": "whole section 30-%s (%d nodes, %d blocks), one pass per open dimension with that "
+                       "dimension's child section resident" % ("".join("%d%d" % c for c in counts), n_out, int(np.prod(nb))),
+           "output_GB": 64 * 4096 * int(np.prod(nb)) / 1e9, "passes": passes, "kernel_ms": kernel_ms,
+           "superpositions_per_s": n_out / (kernel_ms / 1e3),
+           "algorithmic_GBps": alg / (kernel_ms / 1e3) / 1e9,
+           "roofline_frac": alg / (kernel_ms / 1e3) / 1e9 / HBM_PEAK_GBPS,
+           # the same time against the fused form's bytes (each node written once, each child
+           # node read once per line: 64 B x (2,588,774,400 + 3,428,...) = 385 GB)
+           "fused_form_algorithmic_GBps": 64 * sum(whole_section_nodes(P, counts)) / (kernel_ms / 1e3) / 1e9,
+           "wall_s_including_fills_plans_and_checks": wall,
+           "checked_blocks": [list(b) for b in samples], "mismatches": mism}
+    log("section 30 whole: %s" % {k: v for k, v in res.items() if k != "passes"})
+    return res
 
 
 def whole_section_nodes(P, counts):
@@ -444,7 +552,7 @@ def slices_bench(P, hip, budget=None, slice32=False):
     return out
 
 
-def mid_bench(P, hip, n_boards=4096, seed=2):
+def mid_bench(P, hip, n_boards=4096, seed=2, tune=True, plain=True):
     """Batched midsolve: n seeded boards with 18..30 stones in one work-queue launch
     (autotuned tile and workgroups) against one plain pe_midsolve_internal call per board;
     every batched result is compared with the plain one.  Also the 26-stone single board."""
@@ -461,10 +569,11 @@ def mid_bench(P, hip, n_boards=4096, seed=2):
     out["single_26_stones_ms"] = float(np.median(lat))
     boards = seeded_boards(n_boards, seed, 18, 30)
     sample = boards[:512]
-    t0 = time.time()
-    tuned = P.mid_autotune(sample)
-    out["autotune"] = dict(tuned, seconds=time.time() - t0, boards=len(sample))
-    log("midsolve autotune: %s" % out["autotune"])
+    if tune:
+        t0 = time.time()
+        tuned = P.mid_autotune(sample)
+        out["autotune"] = dict(tuned, seconds=time.time() - t0, boards=len(sample))
+        log("midsolve autotune: %s" % out["autotune"])
     t0 = time.time()
     plan = P.MidPlan(boards)
     info = plan.info()
@@ -485,49 +594,54 @@ def mid_bench(P, hip, n_boards=4096, seed=2):
     plan.close()
     batched_s = min(ms) / 1e3
     log("midsolve batched: %s ms" % ms)
-    # plain: one synchronous launch sequence per board
-    plain = np.zeros_like(res)
-    t0 = time.perf_counter()
-    for i, b in enumerate(boards):
-        P.midsolve_internal_raw(b, plain[i])
-    plain_s = time.perf_counter() - t0
-    log("midsolve plain: %.2f s" % plain_s)
     spots = np.array([board_spots(b) for b in boards])
-    mism = int(np.sum(np.any(plain != res, axis=(1, 2)))) + int(np.sum(np.any(res2 != res, axis=(1, 2))))
+    mism = int(np.sum(np.any(res2 != res, axis=(1, 2))))
+    plain_s = None
+    if plain:
+        # plain: one synchronous launch sequence per board
+        pres = np.zeros_like(res)
+        t0 = time.perf_counter()
+        for i, b in enumerate(boards):
+            P.midsolve_internal_raw(b, pres[i])
+        plain_s = time.perf_counter() - t0
+        log("midsolve plain: %.2f s" % plain_s)
+        mism += int(np.sum(np.any(pres != res, axis=(1, 2))))
     work = 0
     from math import comb
     for sp in spots:
         sp = int(sp)
         work += 2 * 16 * sum(comb(sp, n // 2) * comb(sp - n // 2, n - n // 2) * (sp - n + 1) for n in range(sp + 1))
-    # small boards alone (24..30 stones, 6..12 empty cells): launch overhead dominates the
-    # plain path, the batched kernel solves them one workgroup per board
-    small = seeded_boards(n_boards, seed + 1, 24, 30)
-    plan = P.MidPlan(small)
-    plan.run(stream.ptr)
-    sres = plan.results(stream.ptr)
-    ev0.record(stream)
-    plan.run(stream.ptr)
-    ev1.record(stream)
-    ev1.synchronize()
-    small_ms = ev0.elapsed_ms(ev1)
-    plan.close()
-    splain = np.zeros_like(sres)
-    t0 = time.perf_counter()
-    for i, b in enumerate(small):
-        P.midsolve_internal_raw(b, splain[i])
-    small_plain_s = time.perf_counter() - t0
-    out["small_boards"] = {"workload": "%d seeded boards, 24..30 stones (6..12 empty cells)" % n_boards,
-                           "batched_boards_per_s": n_boards / (small_ms / 1e3), "batched_ms": small_ms,
-                           "plain_boards_per_s": n_boards / small_plain_s,
-                           "batched_over_plain": small_plain_s / (small_ms / 1e3),
-                           "mismatches": int(np.sum(np.any(splain != sres, axis=(1, 2))))}
-    log("midsolve small boards: %s" % out["small_boards"])
+    if plain:
+        # small boards alone (24..30 stones, 6..12 empty cells): launch overhead dominates the
+        # plain path, the batched kernel solves them one workgroup per board
+        small = seeded_boards(n_boards, seed + 1, 24, 30)
+        plan = P.MidPlan(small)
+        plan.run(stream.ptr)
+        sres = plan.results(stream.ptr)
+        ev0.record(stream)
+        plan.run(stream.ptr)
+        ev1.record(stream)
+        ev1.synchronize()
+        small_ms = ev0.elapsed_ms(ev1)
+        plan.close()
+        splain = np.zeros_like(sres)
+        t0 = time.perf_counter()
+        for i, b in enumerate(small):
+            P.midsolve_internal_raw(b, splain[i])
+        small_plain_s = time.perf_counter() - t0
+        out["small_boards"] = {"workload": "%d seeded boards, 24..30 stones (6..12 empty cells)" % n_boards,
+                               "batched_boards_per_s": n_boards / (small_ms / 1e3), "batched_ms": small_ms,
+                               "plain_boards_per_s": n_boards / small_plain_s,
+                               "batched_over_plain": small_plain_s / (small_ms / 1e3),
+                               "mismatches": int(np.sum(np.any(splain != sres, axis=(1, 2))))}
+        log("midsolve small boards: %s" % out["small_boards"])
+    floor_s = work / (HBM_PEAK_GBPS * 1e9)
     out.update({
         "config": {"workload": "%d seeded boards, 18..30 stones (6..18 empty cells)" % n_boards,
                    "spots_histogram": {int(k): int(v) for k, v in zip(*np.unique(spots, return_counts=True))}},
         "batched_boards_per_s": n_boards / batched_s, "batched_ms": min(ms), "batched_ms_runs": ms,
-        "plain_boards_per_s": n_boards / plain_s, "plain_s": plain_s,
-        "batched_over_plain": plain_s / batched_s,
+        "plain_boards_per_s": n_boards / plain_s if plain_s else None, "plain_s": plain_s,
+        "batched_over_plain": plain_s / batched_s if plain_s else None,
         "mismatches": mism, "plan": info,
         "algorithmic_GBps": work / batched_s / 1e9,
         # HBM bound of the mix: its algorithmic bytes (32 B per position read or written, both
@@ -537,8 +651,8 @@ def mid_bench(P, hip, n_boards=4096, seed=2):
                      "frac": work / batched_s / 1e9 / HBM_PEAK_GBPS, "traffic": mid_traffic(),
                      "kernel": "batched plan (mid_level1_kernel, mid_boards_kernel, mid_prep_kernel)",
                      "bytes_per_run": work, "run_ms": batched_s * 1e3},
-        "hbm_floor_s": work / (HBM_PEAK_GBPS * 1e9),
-        "max_batched_over_plain_at_hbm_peak": plain_s / (work / (HBM_PEAK_GBPS * 1e9)),
+        "hbm_floor_s": floor_s,
+        "max_batched_over_plain_at_hbm_peak": plain_s / floor_s if plain_s else None,
     })
     return out
 
@@ -652,11 +766,17 @@ def main():
     ap.add_argument("--no-end", action="store_true", help="skip the end-engine measurement (profiling the others)")
     ap.add_argument("--mid-boards", type=int, default=4096, help="boards in the batched midsolve measurement")
     ap.add_argument("--no-section30", dest="section30", action="store_false",
-                    help="skip the slice-30 section box measurement")
+                    help="skip the whole slice-30 section measurement")
     ap.add_argument("--no-slices", dest="slices", action="store_false",
                     help="skip the whole-slice (35, 34, 33) measurement on real data")
     ap.add_argument("--no-line-kernel", dest="line_kernel", action="store_false",
                     help="skip the line-pass (schedule_compute_line form) comparison box")
+    ap.add_argument("--launch", type=int, default=-1,
+                    help="fused launch form: -1 autotune (default), 0 grid, n queue with n workgroups per CU")
+    ap.add_argument("--no-mid-tune", dest="mid_tune", action="store_false",
+                    help="skip the midsolve autotuner (built-in defaults)")
+    ap.add_argument("--no-mid-plain", dest="mid_plain", action="store_false",
+                    help="skip the plain per-board midsolve comparison and the small-board leg")
     ap.add_argument("--no-slice32", dest="slice32", action="store_false",
                     help="skip slice 32 (computed in groups with regenerated children, ~13 s)")
     args = ap.parse_args()
@@ -695,7 +815,10 @@ def main():
 
         stream = hip.Stream()
         launch = None
-        if args.mode == 0:
+        if args.mode == 0 and args.launch >= 0:
+            plan.set_launch(args.launch)
+            launch = {"choice": "grid" if args.launch == 0 else "queue, %d workgroups per CU" % args.launch}
+        elif args.mode == 0:
             # launch form: static grid against the persistent queue-fed grid, fastest kept
             t0 = time.time()
             forms, choice = plan.autotune(stream.ptr)
@@ -772,7 +895,7 @@ def main():
     if not args.no_end and args.slices:
         result["slices"] = slices_bench(P, hip, slice32=args.slice32)
     if not args.no_mid:
-        result["midsolve"] = mid_bench(P, hip, args.mid_boards)
+        result["midsolve"] = mid_bench(P, hip, args.mid_boards, tune=args.mid_tune, plain=args.mid_plain)
     if not args.no_serve:
         result["serving"] = serve_bench(P)
     if not args.no_cpu and not args.no_end:

@@ -166,23 +166,26 @@ __host__ __device__ __forceinline__ uint64_t mix64(uint64_t z) {
   return z ^ (z >> 31);
 }
 
+// One thread per (node, word), grid-stride: a dispatch counts at most 2^32 work-items, and a
+// large slab has more (node, word) pairs than that.
 __global__ void fill_seeded_kernel(uint64_t* __restrict__ nodes, int64_t count, uint64_t seed, int64_t first) {
-  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;   // (node, word)
-  if (t >= count * 4) return;
-  const int64_t node = t >> 2;
-  const int w = int(t & 3);
   const uint64_t key = mix64(seed);
-  const uint64_t ctr = (uint64_t(first + node) * 4 + uint64_t(w)) * 32;
-  // p = 0.1110011001100110 in binary, consumed from the last digit to the first
-  const uint32_t digits = 0xE666;   // bit 15 = first binary digit
-  uint64_t known = 0;
-  for (int k = 0; k < 16; k++) {
-    const uint64_t r = mix64(key ^ (ctr + uint64_t(k)));
-    known = (digits >> k & 1) ? (known | r) : (known & r);
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < count * 4; t += stride) {
+    const int64_t node = t >> 2;
+    const int w = int(t & 3);
+    const uint64_t ctr = (uint64_t(first + node) * 4 + uint64_t(w)) * 32;
+    // p = 0.1110011001100110 in binary, consumed from the last digit to the first
+    const uint32_t digits = 0xE666;   // bit 15 = first binary digit
+    uint64_t known = 0;
+    for (int k = 0; k < 16; k++) {
+      const uint64_t r = mix64(key ^ (ctr + uint64_t(k)));
+      known = (digits >> k & 1) ? (known | r) : (known & r);
+    }
+    const uint64_t win = mix64(key ^ (ctr + 16)) & known;
+    nodes[8 * node + w] = win;
+    nodes[8 * node + 4 + w] = known;
   }
-  const uint64_t win = mix64(key ^ (ctr + 16)) & known;
-  nodes[8 * node + w] = win;
-  nodes[8 * node + 4 + w] = known;
 }
 
 }  // namespace
@@ -268,8 +271,7 @@ void plan_destroy(EndPlan* plan) {
 pe_status launch_fill_seeded(void* d_nodes, int64_t nodes, uint64_t seed, int64_t first, hipStream_t stream) {
   if (nodes <= 0) return PE_OK;
   const int64_t threads = nodes * 4;
-  const int64_t blocks = (threads + 255) / 256;
-  if (blocks > 0x7fffffff) { set_error("fill too large"); return PE_ERR_CAPACITY; }
+  const int64_t blocks = std::min<int64_t>((threads + 255) / 256, int64_t(1) << 20);   // <= 2^28 work-items
   hipLaunchKernelGGL(fill_seeded_kernel, dim3(unsigned(blocks)), dim3(256), 0, stream,
                      static_cast<uint64_t*>(d_nodes), nodes, seed, first);
   PE_HIP(hipGetLastError(), "fill_seeded_kernel launch");

@@ -80,6 +80,7 @@ __device__ __forceinline__ void g3(const NodeFields& F, int k, int a, int fa, in
 
 constexpr int kThreads = 256;
 constexpr int kParts = 4096 / kThreads;
+constexpr int kChunk = 1 << 19;   // blocks per grid launch: 2^31 work-items
 
 // One output block of 4096 nodes, computed by 16 workgroups (`part` = which 256 nodes).
 // S35: blocks of 35-stone sections, whose one move fills the board (no child data).
@@ -242,6 +243,7 @@ void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __r
                          const SecDev* __restrict__ secs, const uint64_t* __restrict__ fields,
                          const uint16_t* __restrict__ movetab, const uint8_t* __restrict__ refl,
                          int nblocks, int blocks_per_xcd) {
+  // `blocks` and `nblocks` describe this launch's chunk of the plan's blocks
   // Workgroups are dispatched to the 8 XCDs round-robin: workgroup i runs on XCD i % 8.
   // XCD x takes the contiguous run of blocks [x * blocks_per_xcd, ...), so the blocks that
   // share child lines (neighbours along the last axes) and the two halves of a 128-byte line
@@ -692,11 +694,16 @@ pe_status fused_plan_run(const FusedPlan& plan, hipStream_t stream) {
     PE_HIP(hipGetLastError(), "fused_blocks_queue_kernel launch");
     return PE_OK;
   }
-  const int64_t grid = int64_t(per_xcd) * kParts * 8;
-  if (grid > 0x7fffffff) { set_error("fused plan too large"); return PE_ERR_CAPACITY; }
-  hipLaunchKernelGGL(plan.s35 ? fused_blocks_kernel<true> : fused_blocks_kernel<false>, dim3(unsigned(grid)),
-                     dim3(kThreads), 0, stream, blocks, lines, secs, T.fields, moves, refl, plan.nblocks, per_xcd);
-  PE_HIP(hipGetLastError(), "fused_blocks_kernel launch");
+  // a dispatch counts at most 2^32 work-items (4096 per block): plans with more blocks run
+  // as consecutive launches of at most kChunk blocks, each spread over the XCDs as above
+  for (int b0 = 0; b0 < plan.nblocks; b0 += kChunk) {
+    const int n = std::min(kChunk, plan.nblocks - b0);
+    const int px = (n + 7) / 8;
+    hipLaunchKernelGGL(plan.s35 ? fused_blocks_kernel<true> : fused_blocks_kernel<false>,
+                       dim3(unsigned(int64_t(px) * kParts * 8)), dim3(kThreads), 0, stream, blocks + b0, lines, secs,
+                       T.fields, moves, refl, n, px);
+    PE_HIP(hipGetLastError(), "fused_blocks_kernel launch");
+  }
   return PE_OK;
 }
 

@@ -218,6 +218,21 @@ def test_compute_line_accumulating_plan_jobs_slice35():
     plan.close()
 
 
+def test_seeded_fill_past_the_dispatch_work_item_limit():
+    """A fill of more than 2^30 nodes (more than 2^32 (node, word) pairs, the most one dispatch
+    counts) writes the last nodes too: the slice-30 section's largest child slab has 1.31e9."""
+    free, _ = hip.mem_info()
+    n = (1 << 30) + (1 << 24)
+    if 64 * n > 0.8 * free:
+        pytest.skip("needs %.0f GB of device memory" % (64 * n / 1e9))
+    buf = hip.Buffer(64 * n)
+    P.fill_seeded(buf.ptr, n, 4242)
+    hip.synchronize()
+    for first in (0, n - 3000, (1 << 30) - 1000):
+        got = buf.download(np.zeros(8 * 3000, dtype=np.uint64), offset=64 * first)
+        assert np.array_equal(got, seeded_nodes(3000, 4242, first)), first
+
+
 def test_seeded_fill_matches_host_generator():
     n = 5000
     buf = hip.Buffer(64 * n)

@@ -1,0 +1,54 @@
+"""Whole sections computed one open dimension at a time (the form the benchmark's slice-30
+section runs in: output resident, one child section at a time, line jobs OR-accumulating
+into slab slots) against the fused slab plan with every child resident, on every block, and
+against the oracle on sampled blocks (pentago/end/compute.cc:219-371 per line,
+pentago/end/block_store.cc:173-229 merge)."""
+import numpy as np
+import pytest
+
+import bench
+import pentago_amd as P
+from pentago_amd import hip
+
+pytestmark = pytest.mark.gpu
+
+SEED = 0x7E570000
+
+
+@pytest.mark.parametrize("counts", [((1, 7), (1, 7), (7, 1), (7, 1)), ((3, 5), (2, 6), (5, 3), (6, 2))])
+def test_section_by_passes_matches_fused_plan_and_oracle(counts):
+    P.init()
+    shape = P.section_shape(counts)
+    nb = [(s + 7) // 8 for s in shape]
+    rng = np.random.default_rng(sum(shape))
+    samples = [tuple(int(rng.integers(n)) for n in nb) for _ in range(2)] + [tuple(n - 1 for n in nb), (0, 0, 0, 0)]
+    print("section", counts, "shape", shape, "samples", samples, flush=True)
+    stream = hip.Stream()
+    out, nb, passes, expect = bench.section_by_passes(P, hip, counts, samples, stream, seed_base=SEED)
+    print("passes", passes, flush=True)
+    # the fused plan over the whole section, every child section resident with the same data
+    kids = []
+    for d in range(4):
+        if sum(counts[d]) >= 9:
+            continue
+        child = P.line_describe(P.make_line(counts, d, (0, 0, 0))).standard_child.counts
+        if any(c == child for c, _ in kids):
+            continue
+        cnb = [(s + 7) // 8 for s in P.section_shape(child)]
+        buf = hip.Buffer(64 * 4096 * int(np.prod(cnb)))
+        P.fill_seeded(buf.ptr, 4096 * int(np.prod(cnb)), SEED + sum(c * 10 ** i for i, c in enumerate(sum(child, ()))))
+        kids.append((child, (cnb, buf)))
+    fused = hip.Buffer(64 * 4096 * int(np.prod(nb)))
+    plan = P.Plan.slab(P.Slab.of(counts, [0] * 4, nb, fused.ptr),
+                       [P.Slab.of(c, [0] * 4, cnb, buf.ptr) for c, (cnb, buf) in kids], 0)
+    plan.run()
+    hip.synchronize()
+    plan.close()
+    a = out.download(np.zeros(8 * 4096 * int(np.prod(nb)), dtype=np.uint64)).reshape(-1, 4096 * 8)
+    b = fused.download(np.zeros(8 * 4096 * int(np.prod(nb)), dtype=np.uint64)).reshape(-1, 4096 * 8)
+    for idx, blk in enumerate(np.ndindex(*nb)):
+        m = 8 * int(np.prod([min(8, shape[x] - 8 * blk[x]) for x in range(4)]))
+        assert np.array_equal(a[idx, :m], b[idx, :m]), ("passes vs fused", blk)
+    for blk, e in expect.items():
+        idx = bench.slab_index(nb, blk)
+        assert np.array_equal(a[idx, :e.size], e), ("passes vs oracle", blk)
