@@ -287,6 +287,85 @@ def line_kernel_bench(P, hip, steps, k=8):
     return out
 
 
+def line_dropin_bench(P, hip, n_lines=256, nthreads=8):
+    """The schedule_compute_line drop-in (pe_end_schedule_line) driven the way the
+    reference's slice loop drives schedule_compute_line (pentago/mpi/flow.cc:465-494): whole
+    block lines of section (3,3)^4 in the reference's line buffers (input block k at node
+    k * (block_stride + 1)), issued from 8 host threads onto their own streams, each call
+    returning at once, one wakeup per line.  Lines/s from the first call to the last wakeup,
+    the host time per call, and the node-line rate."""
+    import ctypes as C
+    import threading
+    shape = P.section_shape(SECTION)
+    nb = [(s + 7) // 8 for s in shape]
+    rng = np.random.default_rng(7)
+    jobs = []
+    node_lines = 0
+    for k in range(n_lines):
+        d = k % 4
+        base = tuple(int(rng.integers(nb[a])) for a in range(4) if a != d)
+        pl = P.make_line(SECTION, d, base)
+        info = P.line_describe(pl)
+        din = hip.Buffer(64 * info.input_nodes)
+        P.fill_seeded(din.ptr, info.input_nodes, 77 + k)
+        dout = hip.Buffer(64 * info.output_nodes)
+        jobs.append((pl, din, dout))
+        node_lines += info.output_nodes
+    hip.synchronize()
+    streams = [hip.Stream() for _ in range(nthreads)]
+    lock = threading.Lock()
+    state = {"woken": 0}
+    all_done = threading.Event()
+
+    def wakeup(ctx):
+        with lock:
+            state["woken"] += 1
+            if state["woken"] == len(jobs):
+                all_done.set()
+
+    cb = P.WAKEUP(wakeup)
+    # the first line of each (section, dimension) builds that pair's cached tables
+    for d in range(4):
+        pl, din, dout = jobs[d]
+        P._check(P.lib().pe_end_schedule_line(C.byref(pl), din.ptr, dout.ptr, streams[0].ptr, None, None),
+                 "pe_end_schedule_line")
+    streams[0].synchronize()
+    call_s = [0.0] * nthreads
+    errors = []
+
+    def issue(t):
+        try:
+            for k in range(t, len(jobs), nthreads):
+                pl, din, dout = jobs[k]
+                t0 = time.perf_counter()
+                st = P.lib().pe_end_schedule_line(C.byref(pl), din.ptr, dout.ptr, streams[t].ptr, cb, k + 1)
+                call_s[t] += time.perf_counter() - t0
+                if st:
+                    raise RuntimeError(P.lib().pe_last_error())
+        except Exception as e:
+            errors.append(e)
+
+    t0 = time.perf_counter()
+    threads = [threading.Thread(target=issue, args=(t,)) for t in range(nthreads)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    ok = all_done.wait(120)
+    dt = time.perf_counter() - t0
+    for s in streams:
+        s.synchronize()
+    del cb
+    if errors or not ok:
+        return {"error": str(errors[0]) if errors else "wakeups missing"}
+    out = {"workload": "%d whole block lines of section 24-(3,3)^4 (all four dimensions, random bases), reference "
+                       "line buffers, %d host threads, one stream each" % (len(jobs), nthreads),
+           "lines_per_s": len(jobs) / dt, "node_lines_per_s": node_lines / dt, "seconds": dt,
+           "host_us_per_call": 1e6 * sum(call_s) / len(jobs), "wakeups": state["woken"]}
+    log("line drop-in: %s" % out)
+    return out
+
+
 def slab_index(nb, b):
     idx = 0
     for a in range(4):
@@ -892,6 +971,7 @@ def main():
         result["section30"] = section30_bench(P, hip, args.steps)
     if not args.no_end and args.line_kernel:
         result["line_kernel"] = line_kernel_bench(P, hip, args.steps)
+        result["line_dropin"] = line_dropin_bench(P, hip)
     if not args.no_end and args.slices:
         result["slices"] = slices_bench(P, hip, slice32=args.slice32)
     if not args.no_mid:
