@@ -289,6 +289,12 @@ pe_status mid_solve_noalloc(const pe_high_board& hb, pe_halfsupers* results, hip
   const int steps = I.spots + 1;
   d.prefix[1] = sz.set0 + sz.wins1 + sz.set1p;
   for (int t = 0; t < steps; t++) d.prefix[2 * (t + 1) + 1] = I.positions[I.spots - t];
+  // every way out waits for the work queued on the stream: the descriptors are read from
+  // this frame, and the next call (on any stream) reuses the array
+  struct Drain {
+    hipStream_t s;
+    ~Drain() { (void)hipStreamSynchronize(s); }
+  } drain{stream};
   PE_HIP(hipMemcpyAsync(base, &d, sizeof d, hipMemcpyHostToDevice, stream), "hipMemcpyAsync(descriptors)");
   const auto* dboard = reinterpret_cast<const MidBoardDev*>(base);
   const auto* dprefix = reinterpret_cast<const int64_t*>(base + offsetof(NoAllocDesc, prefix));
