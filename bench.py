@@ -373,62 +373,52 @@ def slab_index(nb, b):
     return idx
 
 
-def section_by_passes(P, hip, counts, samples, stream, seed_base=0x5EC30000):
-    """A whole section computed one open dimension at a time: the output (every block, one
-    4096-node slot each) stays resident, and each pass holds only that dimension's child
-    section (seeded on the device, not timed; a child shared by two dimensions is filled
-    once).  A pass runs every line of its dimension as a whole-line job on the fused kernel
-    (pe_end_plan_create), the later passes OR-accumulating into the output (the block store's
-    merge, pentago/end/block_store.cc:173-229); each pass runs twice (OR-accumulating twice
-    leaves the same output) and the second run is timed.  For every sampled block the oracle's
-    line through it is computed from child data regenerated on the host from the seed.
+def section_by_passes(P, hip, counts, samples, stream, seed_base=0x5EC30000, groups=None, headroom=0.92):
+    """A whole section computed in passes over subsets of its open dimensions: the output
+    (every block, one 4096-node slot each) stays resident, and each pass holds only its
+    dimensions' child sections (seeded on the device, not timed; a child shared by two
+    dimensions is held once).  By default the open dimensions are grouped greedily, in order,
+    into as few passes as fit beside the output in `headroom` of the free device memory.  A
+    pass is one fused launch with only its dimensions open (pe_end_slabs_plan_create_dims),
+    the later passes OR-accumulating into the output (the block store's merge,
+    pentago/end/block_store.cc:173-229); each pass runs twice (OR-accumulating twice leaves the
+    same output) and the second run is timed.  For every sampled block the oracle's line
+    through it is computed from child data regenerated on the host from the seed.
     Returns (output buffer, block counts, passes, {block: expected nodes})."""
     from oracle import oracle as O
     from tests.seeded import seeded_nodes
     shape = P.section_shape(counts)
     nb = [(s + 7) // 8 for s in shape]
     n_out = int(np.prod(shape))
-    out = hip.Buffer(64 * 4096 * int(np.prod(nb)))
+    out_bytes = 64 * 4096 * int(np.prod(nb))
+    dims = [d for d in range(4) if sum(counts[d]) < 9]
+    child = {}
+    for d in dims:
+        child[d] = P.line_describe(P.make_line(counts, d, tuple(0 for a in range(4) if a != d))).standard_child.counts
+    cnb = {c: [(s + 7) // 8 for s in P.section_shape(c)] for c in child.values()}
+    cbytes = {c: 64 * 4096 * int(np.prod(n)) for c, n in cnb.items()}
+    if groups is None:
+        room = headroom * hip.mem_info()[0] - out_bytes
+        groups = []
+        for d in dims:
+            if groups and sum(cbytes[c] for c in {child[x] for x in groups[-1] + [d]}) <= room:
+                groups[-1].append(d)
+            else:
+                groups.append([d])
+    out = hip.Buffer(out_bytes)
+    out_slab = P.Slab.of(counts, [0] * 4, nb, out.ptr)
     ev0, ev1 = hip.Event(), hip.Event()
     expect = {b: None for b in samples}
     passes = []
-    kid, kid_child = None, None
-    for d in range(4):
-        if sum(counts[d]) >= 9:
-            continue
-        info = P.line_describe(P.make_line(counts, d, tuple(0 for a in range(4) if a != d)))
-        child = info.standard_child.counts
-        cshape = P.section_shape(child)
-        cnb = [(s + 7) // 8 for s in cshape]
-        seed = seed_base + sum(c * 10 ** i for i, c in enumerate(sum(child, ())))
-        if child != kid_child:
-            kid = None
-            kid = hip.Buffer(64 * 4096 * int(np.prod(cnb)))
-            kid_child = child
-            P.fill_seeded(kid.ptr, 4096 * int(np.prod(cnb)), seed)
+    for group in groups:
+        kids = {}
+        for c in {child[d] for d in group}:
+            buf = hip.Buffer(cbytes[c])
+            P.fill_seeded(buf.ptr, 4096 * int(np.prod(cnb[c])), seed_base + sum(v * 10 ** i for i, v in enumerate(sum(c, ()))))
+            kids[c] = buf
         t0 = time.time()
-        jobs = []
-        others = [a for a in range(4) if a != d]
-        for base in np.ndindex(*[nb[a] for a in others]):
-            base = tuple(int(x) for x in base)
-            line = P.make_line(counts, d, base)
-            li = P.line_describe(line)
-            cd = li.child_dimension
-            in_offs = []
-            for k in range(li.input_blocks):
-                cb = list(li.first_child_block)
-                cb[cd] = k
-                in_offs.append(4096 * slab_index(cnb, cb))
-            out_offs = []
-            for k in range(li.length):
-                b = [0] * 4
-                for a, x in zip(others, base):
-                    b[a] = x
-                b[d] = k
-                out_offs.append(4096 * slab_index(nb, b))
-            jobs.append({"line": line, "input": kid.ptr, "output": out.ptr, "in_offsets": in_offs,
-                         "out_offsets": out_offs, "accumulate": bool(passes)})
-        plan = P.Plan(jobs)
+        plan = P.Plan.slab_dims(out_slab, [P.Slab.of(c, [0] * 4, cnb[c], b.ptr) for c, b in kids.items()],
+                                sum(1 << d for d in group), accumulate=bool(passes))
         build_s = time.time() - t0
         hip.synchronize()
         ms = []
@@ -439,36 +429,44 @@ def section_by_passes(P, hip, counts, samples, stream, seed_base=0x5EC30000):
             ev1.synchronize()
             ms.append(ev0.elapsed_ms(ev1))
         plan.close()
-        passes.append({"dimension": d, "child": "".join("%d%d" % c for c in child), "lines": len(jobs),
-                       "child_GB": 64 * 4096 * int(np.prod(cnb)) / 1e9, "ms": ms[1], "first_run_ms": ms[0],
+        # algorithmic bytes: every child node of the pass's lines once, the output written once
+        # (and read once more by every pass after the first)
+        child_nodes = sum(int(np.prod(P.section_shape(child[d]))) for d in group)
+        passes.append({"dimensions": group, "children": sorted("".join("%d%d" % x for x in c) for c in kids),
+                       "child_GB": sum(cbytes[c] for c in kids) / 1e9, "ms": ms[1], "first_run_ms": ms[0],
                        "plan_build_s": build_s,
-                       "algorithmic_bytes": 64 * (int(np.prod(cshape)) + (2 if passes else 1) * n_out)})
-        log("section %s, dimension %d: %s" % ("".join("%d%d" % c for c in counts), d, passes[-1]))
-        for blk in samples:
-            base = tuple(blk[a] for a in range(4) if a != d)
-            ol = O.make_line(counts, d, base)
-            oi = O.line_describe(ol)
-            inp = np.zeros(8 * oi.input_nodes, dtype=np.uint64)
-            for kk in range(oi.input_blocks):
-                cb = list(oi.first_child_block)
-                cb[oi.child_dim] = kk
-                bn = int(np.prod([min(8, cshape[a] - 8 * cb[a]) for a in range(4)]))
-                off = 8 * kk * (oi.block_stride + 1)
-                inp[off:off + 8 * bn] = seeded_nodes(bn, seed, 4096 * slab_index(cnb, cb))
-            line_out = np.zeros(8 * oi.output_nodes, dtype=np.uint64)
-            O.compute_lines_threaded([ol], [inp], [line_out], baseline_threads())
-            bn = int(np.prod([min(8, shape[a] - 8 * blk[a]) for a in range(4)]))
-            part = line_out[8 * oi.block_stride * blk[d]:8 * oi.block_stride * blk[d] + 8 * bn]
-            expect[blk] = part if expect[blk] is None else expect[blk] | part
-    kid = None
+                       "algorithmic_bytes": 64 * (child_nodes + (2 if passes else 1) * n_out)})
+        log("section %s, dimensions %s: %s" % ("".join("%d%d" % c for c in counts), group, passes[-1]))
+        for d in group:
+            c = child[d]
+            seed = seed_base + sum(v * 10 ** i for i, v in enumerate(sum(c, ())))
+            cshape = P.section_shape(c)
+            for blk in samples:
+                base = tuple(blk[a] for a in range(4) if a != d)
+                ol = O.make_line(counts, d, base)
+                oi = O.line_describe(ol)
+                inp = np.zeros(8 * oi.input_nodes, dtype=np.uint64)
+                for kk in range(oi.input_blocks):
+                    cb = list(oi.first_child_block)
+                    cb[oi.child_dim] = kk
+                    bn = int(np.prod([min(8, cshape[a] - 8 * cb[a]) for a in range(4)]))
+                    off = 8 * kk * (oi.block_stride + 1)
+                    inp[off:off + 8 * bn] = seeded_nodes(bn, seed, 4096 * slab_index(cnb[c], cb))
+                line_out = np.zeros(8 * oi.output_nodes, dtype=np.uint64)
+                O.compute_lines_threaded([ol], [inp], [line_out], baseline_threads())
+                bn = int(np.prod([min(8, shape[a] - 8 * blk[a]) for a in range(4)]))
+                part = line_out[8 * oi.block_stride * blk[d]:8 * oi.block_stride * blk[d] + 8 * bn]
+                expect[blk] = part if expect[blk] is None else expect[blk] | part
+        kids = None
     return out, nb, passes, expect
 
 
 def section30_bench(P, hip, steps, check_blocks=2):
     """The whole slice-30 section (4,4)(4,3)(3,4)(4,4) (standard form (3,4)(4,4)(4,4)(4,3),
     shape 318 x 160 x 160 x 318, 2,588,774,400 nodes, 640,000 blocks).  The output stays
-    resident (167.8 GB of slab slots); the child sections (84 + 42 + 67 GB) do not fit beside
-    it together, so the section runs one open dimension at a time (section_by_passes).
+    resident (167.8 GB of slab slots); its child sections (84 + 42 + 67 GB, dimensions 1 and 2
+    share one) do not fit beside it together, so the section runs in passes over subsets of
+    its dimensions (section_by_passes: dimension 0, then dimensions 1-3 fused).
     Kernel time from HIP events, plan builds apart; sampled blocks against the oracle."""
     counts = SECTION30
     shape = P.section_shape(counts)
@@ -487,8 +485,8 @@ def section30_bench(P, hip, steps, check_blocks=2):
     del out
     kernel_ms = sum(p["ms"] for p in passes)
     alg = sum(p["algorithmic_bytes"] for p in passes)
-    res = {"workload": "whole section 30-%s (%d nodes, %d blocks), one pass per open dimension with that "
-                       "dimension's child section resident" % ("".join("%d%d" % c for c in counts), n_out, int(np.prod(nb))),
+    res = {"workload": "whole section 30-%s (%d nodes, %d blocks), in passes over subsets of its open "
+                       "dimensions with only those dimensions' child sections resident" % ("".join("%d%d" % c for c in counts), n_out, int(np.prod(nb))),
            "output_GB": 64 * 4096 * int(np.prod(nb)) / 1e9, "passes": passes, "kernel_ms": kernel_ms,
            "superpositions_per_s": n_out / (kernel_ms / 1e3),
            "algorithmic_GBps": alg / (kernel_ms / 1e3) / 1e9,

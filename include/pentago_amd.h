@@ -130,7 +130,7 @@ pe_status pe_end_plan_destroy(pe_end_plan* plan);
  * batched form of the reference's per-microline job pool (pentago/end/compute.cc:378-404).
  * Runs of one plan must not overlap (they share the queue heads). */
 pe_status pe_end_plan_set_launch(pe_end_plan* plan, int queue_wgs_per_cu);
-/* Time the launch forms {grid, 2, 3, 4 workgroups per CU} on this plan (warm-up, then the
+/* Time the launch forms {grid, 2, 3, 4, 5, 6 workgroups per CU} on this plan (warm-up, then the
  * faster of two runs each, on `stream`; the plan's output is recomputed, identical each
  * time), keep the fastest for the plan and as the current device's default for plans created
  * later.  No counterpart in the reference. */
@@ -169,6 +169,16 @@ pe_status pe_end_slab_plan_create(const pe_slab* output, int nchildren, const pe
  * every block of every output in one fused launch. */
 pe_status pe_end_slabs_plan_create(int noutputs, const pe_slab* outputs, int nchildren, const pe_slab* children,
                                    int mode, pe_end_plan** plan);
+
+/* The fused plan of pe_end_slabs_plan_create (mode 0) for a subset of the open dimensions:
+ * bit d of `dims` set = dimension d computed (its moves and its child lines), the others
+ * skipped, so only the computed dimensions' child slabs are needed; with `accumulate` the
+ * result is ORed into the output (the block-store merge, pentago/end/block_store.cc:173-229),
+ * so runs over disjoint subsets covering every open dimension give the full result.  For
+ * sections too large to hold with all their children at once.  Sections with at most 34
+ * stones. */
+pe_status pe_end_slabs_plan_create_dims(int noutputs, const pe_slab* outputs, int nchildren, const pe_slab* children,
+                                        uint32_t dims, int accumulate, pe_end_plan** plan);
 
 /* Synchronous host-buffer convenience: copies the input to the device, computes, copies the
  * output back.  Same layouts as above. */

@@ -153,6 +153,8 @@ def lib():
             "pe_end_plan_run": (C.c_int, [V, V]),
             "pe_end_slab_plan_create": (C.c_int, [P(Slab), C.c_int, P(Slab), C.c_int, P(V)]),
             "pe_end_slabs_plan_create": (C.c_int, [C.c_int, P(Slab), C.c_int, P(Slab), C.c_int, P(V)]),
+            "pe_end_slabs_plan_create_dims": (C.c_int, [C.c_int, P(Slab), C.c_int, P(Slab), C.c_uint32, C.c_int,
+                                                        P(V)]),
             "pe_end_plan_nodes": (C.c_int64, [V]),
             "pe_end_plan_destroy": (C.c_int, [V]),
             "pe_end_plan_set_launch": (C.c_int, [V, C.c_int]),
@@ -308,6 +310,22 @@ class Plan:
         oarr = (Slab * len(outs))(*outs)
         h = C.c_void_p()
         _check(lib().pe_end_slabs_plan_create(len(outs), oarr, n, arr, mode, C.byref(h)), "pe_end_slabs_plan_create")
+        self.handle = h
+        self.nodes = lib().pe_end_plan_nodes(h)
+        return self
+
+    @classmethod
+    def slab_dims(cls, output, children, dims, accumulate=False):
+        """Fused plan over the output slab(s) computing only the open dimensions in the bit
+        mask `dims` (pe_end_slabs_plan_create_dims), OR-accumulating when asked."""
+        self = cls.__new__(cls)
+        outs = list(output) if isinstance(output, (list, tuple)) else [output]
+        n = len(children)
+        arr = (Slab * max(n, 1))(*children)
+        oarr = (Slab * len(outs))(*outs)
+        h = C.c_void_p()
+        _check(lib().pe_end_slabs_plan_create_dims(len(outs), oarr, n, arr, int(dims), int(bool(accumulate)),
+                                                   C.byref(h)), "pe_end_slabs_plan_create_dims")
         self.handle = h
         self.nodes = lib().pe_end_plan_nodes(h)
         return self

@@ -20,7 +20,7 @@ pe_status plan_create(const std::vector<LineHost>& lines, EndPlan* plan);
 pe_status plan_run(const EndPlan& plan, hipStream_t stream);
 void plan_destroy(EndPlan* plan);
 pe_status fused_plan_create(int nout, const pe_slab* outs, int nch, const pe_slab* ch, FusedPlan* plan,
-                            std::string* err);
+                            std::string* err, uint32_t dims = 0xf, int accumulate = 0);
 pe_status fused_plan_run(const FusedPlan& plan, hipStream_t stream);
 void fused_plan_destroy(FusedPlan* plan);
 pe_status slab_line_passes(const pe_slab& out, int nch, const pe_slab* ch, std::vector<std::vector<LineHost>>* passes,
@@ -363,6 +363,32 @@ pe_status pe_end_slabs_plan_create(int noutputs, const pe_slab* outputs, int nch
   PE_GUARD_END
 }
 
+pe_status pe_end_slabs_plan_create_dims(int noutputs, const pe_slab* outputs, int nchildren, const pe_slab* children,
+                                        uint32_t dims, int accumulate, pe_end_plan** out) {
+  PE_GUARD_BEGIN
+  if (noutputs <= 0 || !outputs || !out || nchildren < 0 || (nchildren && !children) || !(dims & 0xf) || dims > 0xf)
+    return fail(PE_ERR_INVALID, "bad slab plan arguments");
+  for (int k = 0; k < noutputs; k++) {
+    if (!outputs[k].data) return fail(PE_ERR_INVALID, "output slab without memory");
+    int sum = 0;
+    for (int q = 0; q < 4; q++) sum += outputs[k].section.counts[q][0] + outputs[k].section.counts[q][1];
+    if (sum >= 35) return fail(PE_ERR_INVALID, "dimension subsets need sections with at most 34 stones");
+  }
+  auto* plan = new pe_end_plan();
+  std::string err;
+  const pe_status s = fused_plan_create(noutputs, outputs, nchildren, children, &plan->fused, &err, dims, accumulate ? 1 : 0);
+  if (s != PE_OK) {
+    fused_plan_destroy(&plan->fused);
+    delete plan;
+    return fail(s, err);
+  }
+  plan->nodes = plan->fused.output_nodes;
+  apply_default_form(plan);
+  *out = plan;
+  return PE_OK;
+  PE_GUARD_END
+}
+
 pe_status pe_end_plan_run(const pe_end_plan* plan, void* stream) {
   PE_GUARD_BEGIN
   if (!plan) return fail(PE_ERR_INVALID, "null plan");
@@ -405,8 +431,8 @@ pe_status pe_end_plan_autotune(pe_end_plan* plan, void* stream, pe_end_tuning* o
   PE_GUARD_BEGIN
   if (!plan) return fail(PE_ERR_INVALID, "null plan");
   const hipStream_t s = static_cast<hipStream_t>(stream);
-  const int32_t forms[] = {0, 2, 3, 4};
-  constexpr int kForms = 4;
+  const int32_t forms[] = {0, 2, 3, 4, 5, 6};
+  constexpr int kForms = 6;
   pe_end_tuning t;
   std::memset(&t, 0, sizeof t);
   hipEvent_t e0 = nullptr, e1 = nullptr;

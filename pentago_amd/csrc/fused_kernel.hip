@@ -81,6 +81,12 @@ __device__ __forceinline__ void g3(const NodeFields& F, int k, int a, int fa, in
 constexpr int kThreads = 256;
 constexpr int kParts = 4096 / kThreads;
 constexpr int kChunk = 1 << 19;   // blocks per grid launch: 2^31 work-items
+// Occupancy: each node's running OR over its dimensions (and then its output words) lives in
+// LDS, so a wave needs ~74 VGPRs and five waves share a SIMD; children are loaded one move
+// at a time.  Measured on the headline launch against keeping them in registers (123 VGPRs,
+// four waves, moves loaded three at a time): 157.4 vs 168.4 ms; two moves at a time 162.5,
+// six waves 159.1.
+constexpr int kWaves = 5;
 
 // One output block of 4096 nodes, computed by 16 workgroups (`part` = which 256 nodes).
 // S35: blocks of 35-stone sections, whose one move fills the board (no child data).
@@ -117,7 +123,11 @@ __device__ __forceinline__ void fused_block(const BlockDev& B, const SecDev& S, 
   // ---- children: per open dimension, AND the moves' children shifted along the child
   //      dimension (OR of complements = complement of AND), then apply the rest of the
   //      local rotation and the rearrangement ----
-  S4 t0 = s4_zero(), t1 = s4_zero();
+  // the node's running OR over the dimensions, then its output, in LDS: its own 8 words
+  // ([word][thread], so a wave's accesses are consecutive 8-byte slots), never shared
+  __shared__ uint64_t sacc[S35 ? 1 : 8][kThreads];
+  if (!S35)
+    for (int w = 0; w < 8; w++) sacc[w][threadIdx.x] = 0;
 #pragma unroll
   for (int d = 0; d < 4; d++) {
     if (S35 || bline[d] < 0) continue;
@@ -139,34 +149,34 @@ __device__ __forceinline__ void fused_block(const BlockDev& B, const SecDev& S, 
     const uint64_t* in = L.input + 8 * L.in_base;
     S4 acc0, acc1;
     for (int i = 0; i < 4; i++) acc0.w[i] = acc1.w[i] = ~uint64_t(0);
-    // moves in groups of three: all loads of a group are issued before any is used
-    for (int m0 = 0; m0 < nm; m0 += 3) {
-      S4 c0[3], c1[3];
-      int ks[3];
-#pragma unroll
-      for (int u = 0; u < 3; u++) {
-        if (m0 + u >= nm) break;
-        const int e = row[m0 + u];
-        const int kk = e & 63;
-        const int64_t cidx = int64_t(kk) * L.in_kstride + (kk == last ? base_last : base) +
-                             int64_t(L.input_stride) * ((e >> 6) & 7);
-        const uint64_t* c = in + 8 * cidx;
-        ks[u] = e >> 9;
-        c0[u] = ld(c);
-        c1[u] = ld(c + 4);
-      }
-#pragma unroll
-      for (int u = 0; u < 3; u++) {
-        if (m0 + u >= nm) break;
-        acc0 = s4_and(acc0, shift_coord_var(c0[u], cd, ks[u]));
-        acc1 = s4_and(acc1, shift_coord_var(c1[u], cd, ks[u]));
-      }
+    // one move at a time: its child's four loads, then its shift
+    for (int m = 0; m < nm; m++) {
+      const int e = row[m];
+      const int kk = e & 63;
+      const int64_t cidx = int64_t(kk) * L.in_kstride + (kk == last ? base_last : base) +
+                           int64_t(L.input_stride) * ((e >> 6) & 7);
+      const uint64_t* c = in + 8 * cidx;
+      const S4 c0 = ld(c), c1 = ld(c + 4);
+      acc0 = s4_and(acc0, shift_coord_var(c0, cd, e >> 9));
+      acc1 = s4_and(acc1, shift_coord_var(c1, cd, e >> 9));
     }
     const int y = local_add(L.inv_local, local_sym);
-    t0 = s4_or(t0, rearrange(shift_local(s4_not(acc0), y), L.inv_global));
-    t1 = s4_or(t1, rearrange(shift_local(s4_not(acc1), y), L.inv_global));
+    const S4 x0 = rearrange(shift_local(s4_not(acc0), y), L.inv_global);
+#pragma unroll
+    for (int w = 0; w < 4; w++) sacc[w][threadIdx.x] |= x0.w[w];
+    const S4 x1 = rearrange(shift_local(s4_not(acc1), y), L.inv_global);
+#pragma unroll
+    for (int w = 0; w < 4; w++) sacc[4 + w][threadIdx.x] |= x1.w[w];
   }
-  const S4 r0 = rmax(t0), r1 = rmax(t1);
+  if (!S35) {   // rmax in place: words 0-3 from the children's [0] halves, 4-7 from [1]
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      S4 t;
+      for (int w = 0; w < 4; w++) t.w[w] = sacc[4 * h + w][threadIdx.x];
+      t = rmax(t);
+      for (int w = 0; w < 4; w++) sacc[4 * h + w][threadIdx.x] = t.w[w];
+    }
+  }
 
   // ---- wins0, wins1 and the wins of every move (pentago/base/superscore.cc:38-96 pattern
   //      groups), one 64-bit word at a time: every reduction except the one over r3 stays
@@ -212,9 +222,11 @@ __device__ __forceinline__ void fused_block(const BlockDev& B, const SecDev& S, 
       w0v.w[k] = w[0];
       w1v.w[k] = w[1];
     } else {
+      // the output words replace the child words they are made from
       const uint64_t imm = w[0] | w[1];
-      o0.w[k] = ((w[2] | r1.w[k]) & ~imm) | (w[0] & ~w[1]);
-      o1.w[k] = (r0.w[k] & ~imm) | w[0];
+      const uint64_t r0k = sacc[k][threadIdx.x], r1k = sacc[4 + k][threadIdx.x];
+      sacc[4 + k][threadIdx.x] = ((w[2] | r1k) & ~imm) | (w[0] & ~w[1]);
+      sacc[k][threadIdx.x] = (r0k & ~imm) | w[0];
     }
     PE_FENCE();
   }
@@ -228,6 +240,12 @@ __device__ __forceinline__ void fused_block(const BlockDev& B, const SecDev& S, 
       o1.w[k] = (nl.w[k] & ~imm) | w0v.w[k];
     }
   }
+  if (!S35)
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      o0.w[k] = sacc[4 + k][threadIdx.x];
+      o1.w[k] = sacc[k][threadIdx.x];
+    }
   uint64_t* o = B.out + 8 * p;
   if (B.accumulate) {   // OR-merge with other dimensions' lines (block_store.cc:173-229)
     o0 = s4_or(o0, ld(o));
@@ -238,7 +256,7 @@ __device__ __forceinline__ void fused_block(const BlockDev& B, const SecDev& S, 
 }
 
 template <bool S35>
-__global__ __launch_bounds__(kThreads)
+__global__ __launch_bounds__(kThreads, kWaves)
 void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __restrict__ lines,
                          const SecDev* __restrict__ secs, const uint64_t* __restrict__ fields,
                          const uint16_t* __restrict__ movetab, const uint8_t* __restrict__ refl,
@@ -261,7 +279,7 @@ void fused_blocks_kernel(const BlockDev* __restrict__ blocks, const LineDev* __r
 // i % 8) and, once that is exhausted, from the other runs in turn; every pop past a run's
 // end fails, so each workgroup stops after at most eight failed pops.
 template <bool S35>
-__global__ __launch_bounds__(kThreads)
+__global__ __launch_bounds__(kThreads, kWaves)
 void fused_blocks_queue_kernel(const BlockDev* __restrict__ blocks, const LineDev* __restrict__ lines,
                                const SecDev* __restrict__ secs, const uint64_t* __restrict__ fields,
                                const uint16_t* __restrict__ movetab, const uint8_t* __restrict__ refl,
@@ -298,7 +316,7 @@ struct LineLaunch {
 // Workgroup i computes part i % 16 of block i / 16 of the line; output block k at node
 // k * block_stride, child block k at node k * (block_stride + 1) (the reference's buffers).
 template <bool S35>
-__global__ __launch_bounds__(kThreads)
+__global__ __launch_bounds__(kThreads, kWaves)
 void fused_line_kernel(const LineLaunch a, const uint64_t* __restrict__ fields,
                        const uint16_t* __restrict__ movetab, const uint8_t* __restrict__ refl) {
   const int k = int(blockIdx.x) / kParts, part = int(blockIdx.x) % kParts;
@@ -331,7 +349,8 @@ bool same_section(const pe_section& a, const pe_section& b) { return !std::memcm
 // Build the lines through an output slab (one per open dimension and cross-section block)
 // with child addressing into the child slabs.
 static pe_status slab_lines(const pe_slab& out, int nch, const pe_slab* ch, std::vector<LineHost>* lines,
-                            std::map<std::tuple<int, int, int, int>, int>* index, std::string* err) {
+                            std::map<std::tuple<int, int, int, int>, int>* index, std::string* err,
+                            uint32_t dims = 0xf) {
   Section s;
   std::memcpy(s.c, out.section.counts, 8);
   int shape[4];
@@ -347,7 +366,7 @@ static pe_status slab_lines(const pe_slab& out, int nch, const pe_slab* ch, std:
     by_section[cs.sig()].push_back(c);
   }
   for (int d = 0; d < 4; d++) {
-    if (s.c[d][0] + s.c[d][1] >= 9) continue;
+    if (s.c[d][0] + s.c[d][1] >= 9 || !(dims >> d & 1)) continue;
     int o[3], k = 0;
     for (int a = 0; a < 4; a++) if (a != d) o[k++] = a;
     for (int x = out.lo[o[0]]; x < out.hi[o[0]]; x++)
@@ -474,11 +493,12 @@ struct SlabBuild {
   std::string err;
 };
 
-static void build_slab_body(const pe_slab& out, int nch, const pe_slab* ch, SlabBuild* r) {
+static void build_slab_body(const pe_slab& out, int nch, const pe_slab* ch, uint32_t dims, int accumulate,
+                            SlabBuild* r) {
   const auto& Tb = tables();
   if (!out.data) { r->err = "output slab without memory"; r->status = PE_ERR_INVALID; return; }
   std::map<std::tuple<int, int, int, int>, int> index;
-  r->status = slab_lines(out, nch, ch, &r->lines, &index, &r->err);
+  r->status = slab_lines(out, nch, ch, &r->lines, &index, &r->err, dims);
   if (r->status != PE_OK) return;
   Section s;
   std::memcpy(s.c, out.section.counts, 8);
@@ -492,7 +512,9 @@ static void build_slab_body(const pe_slab& out, int nch, const pe_slab* ch, Slab
     const int b = rmin_bucket(s.c[a][0], s.c[a][1]);
     sec.rmin_off[a] = Tb.rmin_offsets[b];
     sec.rmin_size[a] = Tb.rmin_offsets[b + 1] - Tb.rmin_offsets[b];
-    sec.field_off[a] = field_pool_off(a, sec.turn, 1, s.c[a][0], s.c[a][1]);
+    // a quadrant's moves count only when its dimension is computed
+    sec.field_off[a] = field_pool_off(a, sec.turn, (dims >> a) & 1, s.c[a][0], s.c[a][1]);
+    if (!(dims >> a & 1)) { sec.move_off[a] = -1; sec.nmoves[a] = 0; }
   }
   int b[4];
   for (b[0] = out.lo[0]; b[0] < out.hi[0]; b[0]++)
@@ -501,12 +523,12 @@ static void build_slab_body(const pe_slab& out, int nch, const pe_slab* ch, Slab
         for (b[3] = out.lo[3]; b[3] < out.hi[3]; b[3]++) {
           BlockDev B;
           B.sec = 0;
-          B.accumulate = 0;
+          B.accumulate = accumulate;
           int n = 1;
           for (int a = 0; a < 4; a++) { B.b[a] = b[a]; n *= std::min(8, sec.rmin_size[a] - 8 * b[a]); }
           for (int d = 0; d < 4; d++) {
             B.line[d] = -1;
-            if (s.c[d][0] + s.c[d][1] >= 9) continue;
+            if (s.c[d][0] + s.c[d][1] >= 9 || !(dims >> d & 1)) continue;
             int o[3], kk = 0;
             for (int a = 0; a < 4; a++) if (a != d) o[kk++] = b[a];
             B.line[d] = index.at(std::make_tuple(d, o[0], o[1], o[2]));
@@ -519,9 +541,9 @@ static void build_slab_body(const pe_slab& out, int nch, const pe_slab* ch, Slab
 
 // Worker-thread entry: no exception leaves the thread (a host allocation failure or a bad
 // index becomes the slab's error status).
-static void build_slab(const pe_slab& out, int nch, const pe_slab* ch, SlabBuild* r) {
+static void build_slab(const pe_slab& out, int nch, const pe_slab* ch, uint32_t dims, int accumulate, SlabBuild* r) {
   try {
-    build_slab_body(out, nch, ch, r);
+    build_slab_body(out, nch, ch, dims, accumulate, r);
   } catch (const std::bad_alloc&) {
     r->status = PE_ERR_CAPACITY;
     r->err = "host memory exhausted while building a fused plan";
@@ -532,7 +554,7 @@ static void build_slab(const pe_slab& out, int nch, const pe_slab* ch, SlabBuild
 }
 
 pe_status fused_plan_create(int nout, const pe_slab* outs, int nch, const pe_slab* ch, FusedPlan* plan,
-                            std::string* err) {
+                            std::string* err, uint32_t dims, int accumulate) {
   DevTables T;
   if (device_tables(&T, err) != PE_OK) return PE_ERR_DEVICE;
   // one plan holds either sections with at most 34 stones or 35-stone sections only
@@ -551,7 +573,7 @@ pe_status fused_plan_create(int nout, const pe_slab* outs, int nch, const pe_sla
   const int nthreads = std::max(1, std::min({nout, int(std::thread::hardware_concurrency()), 16}));
   std::atomic<int> next(0);
   auto work = [&]() {
-    for (int k; (k = next.fetch_add(1)) < nout;) build_slab(outs[k], nch, ch, &builds[size_t(k)]);
+    for (int k; (k = next.fetch_add(1)) < nout;) build_slab(outs[k], nch, ch, dims, accumulate, &builds[size_t(k)]);
   };
   std::vector<std::thread> pool;
   struct Join {   // joins the started threads on every way out, a failed thread start included

@@ -1,6 +1,7 @@
-"""Whole sections computed one open dimension at a time (the form the benchmark's slice-30
-section runs in: output resident, one child section at a time, line jobs OR-accumulating
-into slab slots) against the fused slab plan with every child resident, on every block, and
+"""Whole sections computed in passes over subsets of their open dimensions (the form the
+benchmark's slice-30 section runs in: output resident, only the pass's child sections
+resident, fused launches with the other dimensions closed, later passes OR-accumulating into
+the slab slots) against the fused slab plan with every child resident, on every block, and
 against the oracle on sampled blocks (pentago/end/compute.cc:219-371 per line,
 pentago/end/block_store.cc:173-229 merge)."""
 import numpy as np
@@ -15,8 +16,13 @@ pytestmark = pytest.mark.gpu
 SEED = 0x7E570000
 
 
-@pytest.mark.parametrize("counts", [((1, 7), (1, 7), (7, 1), (7, 1)), ((3, 5), (2, 6), (5, 3), (6, 2))])
-def test_section_by_passes_matches_fused_plan_and_oracle(counts):
+@pytest.mark.parametrize("counts,groups", [
+    (((1, 7), (1, 7), (7, 1), (7, 1)), [[0], [1], [2], [3]]),
+    (((1, 7), (1, 7), (7, 1), (7, 1)), [[0, 3], [1, 2]]),
+    (((3, 5), (2, 6), (5, 3), (6, 2)), [[0], [1, 2, 3]]),
+    (((3, 5), (2, 6), (5, 3), (6, 2)), None),
+])
+def test_section_by_passes_matches_fused_plan_and_oracle(counts, groups):
     P.init()
     shape = P.section_shape(counts)
     nb = [(s + 7) // 8 for s in shape]
@@ -24,7 +30,7 @@ def test_section_by_passes_matches_fused_plan_and_oracle(counts):
     samples = [tuple(int(rng.integers(n)) for n in nb) for _ in range(2)] + [tuple(n - 1 for n in nb), (0, 0, 0, 0)]
     print("section", counts, "shape", shape, "samples", samples, flush=True)
     stream = hip.Stream()
-    out, nb, passes, expect = bench.section_by_passes(P, hip, counts, samples, stream, seed_base=SEED)
+    out, nb, passes, expect = bench.section_by_passes(P, hip, counts, samples, stream, seed_base=SEED, groups=groups)
     print("passes", passes, flush=True)
     # the fused plan over the whole section, every child section resident with the same data
     kids = []
