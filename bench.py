@@ -413,9 +413,8 @@ def section_by_passes(P, hip, counts, samples, stream, seed_base=0x5EC30000, gro
     for group in groups:
         kids = {}
         for c in {child[d] for d in group}:
-            buf = hip.Buffer(cbytes[c])
-            P.fill_seeded(buf.ptr, 4096 * int(np.prod(cnb[c])), seed_base + sum(v * 10 ** i for i, v in enumerate(sum(c, ()))))
-            kids[c] = buf
+            kids[c] = hip.Buffer(cbytes[c])
+            P.fill_seeded(kids[c].ptr, 4096 * int(np.prod(cnb[c])), seed_base + sum(v * 10 ** i for i, v in enumerate(sum(c, ()))))
         t0 = time.time()
         plan = P.Plan.slab_dims(out_slab, [P.Slab.of(c, [0] * 4, cnb[c], b.ptr) for c, b in kids.items()],
                                 sum(1 << d for d in group), accumulate=bool(passes))
@@ -854,6 +853,8 @@ def main():
                     help="skip the midsolve autotuner (built-in defaults)")
     ap.add_argument("--no-mid-plain", dest="mid_plain", action="store_false",
                     help="skip the plain per-board midsolve comparison and the small-board leg")
+    ap.add_argument("--no-line-dropin", dest="line_dropin", action="store_false",
+                    help="skip the line drop-in measurement (lines scheduled from host threads)")
     ap.add_argument("--no-slice32", dest="slice32", action="store_false",
                     help="skip slice 32 (computed in groups with regenerated children, ~13 s)")
     args = ap.parse_args()
@@ -969,6 +970,7 @@ def main():
         result["section30"] = section30_bench(P, hip, args.steps)
     if not args.no_end and args.line_kernel:
         result["line_kernel"] = line_kernel_bench(P, hip, args.steps)
+    if not args.no_end and args.line_dropin:
         result["line_dropin"] = line_dropin_bench(P, hip)
     if not args.no_end and args.slices:
         result["slices"] = slices_bench(P, hip, slice32=args.slice32)

@@ -262,6 +262,9 @@ pe_status make_line(const pe_line& line, LineHost* out, std::string* err) {
   D.i_end = D.rmin_size[dim];
   D.accumulate = 0;
   D.reflected = (st & 4) ? 1 : 0;
+  // unreflected lines: (ig, il) = (-r, r (1,1,1,1)), a coordinate permutation plus a shift of
+  // every coordinate by r, which the fused kernel applies as generated straight-line code
+  D.fast_t = (st & 4) || il != 0x55 * (st & 3) || ig != ((4 - (st & 3)) & 3) ? -1 : ig;
   if (D.child_coord != I.child_dimension) { *err = "internal: moved quadrant does not map to the child dimension"; return PE_ERR_INVALID; }
   // child block strides per output axis (compute.cc:254-271 with the block shape fixed per line)
   {

@@ -53,6 +53,7 @@ struct LineDev {
   int32_t i_begin, i_end;              // range of indices along the line dimension to compute
   int32_t accumulate;                  // OR into the output instead of storing
   int32_t reflected;                   // child standardized through a reflection
+  int32_t fast_t;                      // unreflected: inv_global (0-3, y = -g (1,1,1,1)); -1 otherwise
   // Per output axis a (zero for the line dimension): the child axis c it maps to is
   // perm^-1(a); stride of c inside a child block (full / last block), 8*first_child_block[c],
   // reflection_moves[c] and the offset of c's reflection symmetries.

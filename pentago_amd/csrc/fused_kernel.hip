@@ -36,6 +36,7 @@
 #include "end_device.h"
 #include "super.h"
 #include "tables.h"
+#include "transforms.h"
 
 namespace pe {
 
@@ -160,11 +161,26 @@ __device__ __forceinline__ void fused_block(const BlockDev& B, const SecDev& S, 
       acc0 = s4_and(acc0, shift_coord_var(c0, cd, e >> 9));
       acc1 = s4_and(acc1, shift_coord_var(c1, cd, e >> 9));
     }
-    const int y = local_add(L.inv_local, local_sym);
-    const S4 x0 = rearrange(shift_local(s4_not(acc0), y), L.inv_global);
+    // child frame -> parent frame: X(commute[g][x] + y).  Unreflected lines use one of four
+    // fixed symmetries (straight-line code); reflected ones have a per-node y.
+    S4 x0 = s4_not(acc0), x1 = s4_not(acc1);
+    const int ft = L.fast_t;
+    if (ft == 1) {
+      x0 = line_transform1(x0);
+      x1 = line_transform1(x1);
+    } else if (ft == 2) {
+      x0 = line_transform2(x0);
+      x1 = line_transform2(x1);
+    } else if (ft == 3) {
+      x0 = line_transform3(x0);
+      x1 = line_transform3(x1);
+    } else if (ft < 0) {
+      const int y = local_add(L.inv_local, local_sym);
+      x0 = rearrange(shift_local(x0, y), L.inv_global);
+      x1 = rearrange(shift_local(x1, y), L.inv_global);
+    }
 #pragma unroll
     for (int w = 0; w < 4; w++) sacc[w][threadIdx.x] |= x0.w[w];
-    const S4 x1 = rearrange(shift_local(s4_not(acc1), y), L.inv_global);
 #pragma unroll
     for (int w = 0; w < 4; w++) sacc[4 + w][threadIdx.x] |= x1.w[w];
   }

@@ -68,6 +68,21 @@ PE_HD uint64_t select64(uint64_t a, uint64_t b, uint32_t m) {
   return (a & ~m64) | (b & m64);
 #endif
 }
+// Byte gather from the 8-byte value (hi:lo): byte k of the result is byte sel_k of it
+// (0-3 from lo, 4-7 from hi) or zero (sel_k = 12); v_perm_b32 on the device.
+PE_HD uint32_t pe_perm(uint32_t hi, uint32_t lo, uint32_t sel) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_perm(hi, lo, sel);
+#else
+  const uint64_t v = uint64_t(hi) << 32 | lo;
+  uint32_t r = 0;
+  for (int k = 0; k < 4; k++) {
+    const uint32_t s = sel >> (8 * k) & 0xff;
+    r |= (s < 8 ? uint32_t(v >> (8 * s) & 0xff) : 0u) << (8 * k);
+  }
+  return r;
+#endif
+}
 PE_HD S4 shift3(S4 a, int k) {
   // word rotation by k as two stages of selects (a per-thread index into the words would be
   // lowered to a private-memory array)

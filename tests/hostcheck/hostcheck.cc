@@ -3,6 +3,7 @@
 #include <cstring>
 
 #include "../../pentago_amd/csrc/super.h"
+#include "../../pentago_amd/csrc/transforms.h"
 
 using namespace pe;
 
@@ -19,6 +20,11 @@ void hc_split(const uint64_t* in, int parity, uint64_t* out) {
 }
 void hc_shift_var(int q, int k, const uint64_t* in, uint64_t* out) {
   S4 a; std::memcpy(a.w, in, 32); a = shift_coord_var(a, q, k); std::memcpy(out, a.w, 32);
+}
+void hc_line_transform(int g, const uint64_t* in, uint64_t* out) {
+  S4 a; std::memcpy(a.w, in, 32);
+  a = g == 1 ? line_transform1(a) : g == 2 ? line_transform2(a) : line_transform3(a);
+  std::memcpy(out, a.w, 32);
 }
 void hc_hrmax(const uint64_t* in, uint64_t* out) { H2 h; std::memcpy(h.w, in, 16); h = hrmax(h); std::memcpy(out, h.w, 16); }
 }

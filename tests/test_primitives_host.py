@@ -65,6 +65,25 @@ def test_shift_one_coordinate(hc):
                 assert val(out) == O.transform_super((0, k << (2 * q)), s), (q, k)
 
 
+@pytest.mark.parametrize("g", [1, 2, 3])
+def test_line_transforms_of_unreflected_lines(hc, g):
+    """The generated straight-line transforms equal transform_super((g, y)) of the oracle for
+    the three rotations an unreflected line uses, y = r (1,1,1,1) with r = -g mod 4
+    (pentago/end/compute.cc:69), on every single bit (a permutation is fixed by where each
+    bit goes) and on random supers."""
+    y = 0x55 * ((4 - g) & 3)
+    for j in range(256):
+        out = (C.c_uint64 * 4)()
+        hc.hc_line_transform(g, arr(1 << j, 4), out)
+        assert val(out) == O.transform_super((g, y), 1 << j), (g, j)
+    rng = np.random.default_rng(6 + g)
+    for _ in range(50):
+        s = int.from_bytes(rng.bytes(32), "little")
+        out = (C.c_uint64 * 4)()
+        hc.hc_line_transform(g, arr(s, 4), out)
+        assert val(out) == O.transform_super((g, y), s)
+
+
 def test_super_wins(hc):
     f = O.lib().po_superwin_info
     f.restype = C.POINTER(C.c_uint64)
