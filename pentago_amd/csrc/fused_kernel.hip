@@ -66,13 +66,37 @@ __device__ __forceinline__ void g2(const NodeFields& F, int k, int a, int fa, in
   x[1] = F.w(a, k, 1, fa) & F.w(b, k, 1, fb);
   x[2] = (F.w(a, k, 2, fa) & b0) | (a0 & F.w(b, k, 2, fb));
 }
-__device__ __forceinline__ void g3(const NodeFields& F, int k, int a, int fa, int b, int fb, int c, int fc,
-                                   uint64_t x[3]) {
+// The same terms with three-input logic operations (at least one of the quadrants is lane-
+// varying; products of two wave-uniform quadrants stay scalar).  g2_acc ORs the term into
+// acc; g2_term / g3_term return it.
+__device__ __forceinline__ void g2_acc(const NodeFields& F, int k, int a, int fa, int b, int fb, uint64_t acc[3]) {
+  const uint64_t a0 = F.w(a, k, 0, fa), b0 = F.w(b, k, 0, fb);
+  acc[0] = lop3<kOrAnd>(acc[0], a0, b0);
+  acc[1] = lop3<kOrAnd>(acc[1], F.w(a, k, 1, fa), F.w(b, k, 1, fb));
+  acc[2] = lop3<kOrAnd>(lop3<kOrAnd>(acc[2], F.w(a, k, 2, fa), b0), a0, F.w(b, k, 2, fb));
+}
+__device__ __forceinline__ void g2_term(const NodeFields& F, int k, int a, int fa, int b, int fb, uint64_t x[3]) {
+  const uint64_t a0 = F.w(a, k, 0, fa), b0 = F.w(b, k, 0, fb);
+  x[0] = a0 & b0;
+  x[1] = F.w(a, k, 1, fa) & F.w(b, k, 1, fb);
+  x[2] = lop3<kOrAnd>(F.w(a, k, 2, fa) & b0, a0, F.w(b, k, 2, fb));
+}
+// a2 b0 c0 | a0 b2 c0 | a0 b0 c2, with a (and b when b < 2) wave-uniform
+__device__ __forceinline__ void g3_term(const NodeFields& F, int k, int a, int fa, int b, int fb, int c, int fc,
+                                        uint64_t x[3]) {
   const uint64_t a0 = F.w(a, k, 0, fa), b0 = F.w(b, k, 0, fb), c0 = F.w(c, k, 0, fc);
-  const uint64_t bc = b0 & c0;
-  x[0] = a0 & bc;
-  x[1] = F.w(a, k, 1, fa) & F.w(b, k, 1, fb) & F.w(c, k, 1, fc);
-  x[2] = (F.w(a, k, 2, fa) & bc) | (a0 & ((F.w(b, k, 2, fb) & c0) | (b0 & F.w(c, k, 2, fc))));
+  if (b < 2) {
+    const uint64_t ab0 = a0 & b0, ab2 = (F.w(a, k, 2, fa) & b0) | (a0 & F.w(b, k, 2, fb));
+    x[0] = ab0 & c0;
+    x[1] = (F.w(a, k, 1, fa) & F.w(b, k, 1, fb)) & F.w(c, k, 1, fc);
+    x[2] = lop3<kOrAnd>(ab2 & c0, ab0, F.w(c, k, 2, fc));
+  } else {
+    const uint64_t bc0 = b0 & c0;
+    x[0] = a0 & bc0;
+    x[1] = lop3<kAndAnd>(F.w(a, k, 1, fa), F.w(b, k, 1, fb), F.w(c, k, 1, fc));
+    const uint64_t inner = lop3<kOrAnd>(F.w(b, k, 2, fb) & c0, b0, F.w(c, k, 2, fc));
+    x[2] = lop3<kOrAnd>(F.w(a, k, 2, fa) & bc0, a0, inner);
+  }
 }
 
 // Compiler-only memory fence: keeps the field loads of one word from being hoisted above
@@ -207,13 +231,12 @@ __device__ __forceinline__ void fused_block(const BlockDev& B, const SecDev& S, 
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     uint64_t x[3];
-    g2(F, k, 0, FH, 2, FH, x);
-    for (int t = 0; t < 3; t++) A[t] |= x[t];
+    g2_acc(F, k, 0, FH, 2, FH, A);
     PE_FENCE();
-    g2(F, k, 0, FV, 1, FV, x);
+    g2(F, k, 0, FV, 1, FV, x);   // both quadrants wave-uniform: scalar
     for (int t = 0; t < 3; t++) Bv[t] |= x[t];
     PE_FENCE();
-    g3(F, k, 1, FLO, 0, FAS, 2, FLO, x);
+    g3_term(F, k, 1, FLO, 0, FAS, 2, FLO, x);
     for (int t = 0; t < 3; t++) Cv[t] |= x[t];
     PE_FENCE();
   }
@@ -221,15 +244,15 @@ __device__ __forceinline__ void fused_block(const BlockDev& B, const SecDev& S, 
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     uint64_t w[3], x[3], y[3];
-    g2(F, k, 2, FV, 3, FV, x);
-    g3(F, k, 0, FLO, 2, FAS, 3, FLO, y);
-    for (int t = 0; t < 3; t++) w[t] = red1(A[t] | red0(x[t]) | y[t]) | Cv[t];
+    g2_term(F, k, 2, FV, 3, FV, x);
+    g3_term(F, k, 0, FLO, 2, FAS, 3, FLO, y);
+    for (int t = 0; t < 3; t++) w[t] = red1(lop3<kOr3>(A[t], red0(x[t]), y[t])) | Cv[t];
     PE_FENCE();
-    g2(F, k, 1, FH, 3, FH, x);
-    g3(F, k, 0, FHI, 1, FAS, 3, FHI, y);
-    for (int t = 0; t < 3; t++) w[t] |= red2(Bv[t] | red0(x[t]) | y[t]);
+    g2_term(F, k, 1, FH, 3, FH, x);
+    g3_term(F, k, 0, FHI, 1, FAS, 3, FHI, y);
+    for (int t = 0; t < 3; t++) w[t] |= red2(lop3<kOr3>(Bv[t], red0(x[t]), y[t]));
     PE_FENCE();
-    g3(F, k, 1, FHI, 3, FAS, 2, FHI, x);
+    g3_term(F, k, 1, FHI, 3, FAS, 2, FHI, x);
     for (int t = 0; t < 3; t++) w[t] |= red0(x[t]);
     if (S35) {
       // OR over the moves of super_wins(side0 + move) is w[2] | w[0] exactly: groups through
@@ -239,10 +262,11 @@ __device__ __forceinline__ void fused_block(const BlockDev& B, const SecDev& S, 
       w1v.w[k] = w[1];
     } else {
       // the output words replace the child words they are made from
-      const uint64_t imm = w[0] | w[1];
+      // ((w2 | r1) & ~(w0 | w1)) | (w0 & ~w1) and (r0 & ~(w0 | w1)) | w0 as functions of
+      // (u, w0, w1): truth tables 0x54 and 0xDC
       const uint64_t r0k = sacc[k][threadIdx.x], r1k = sacc[4 + k][threadIdx.x];
-      sacc[4 + k][threadIdx.x] = ((w[2] | r1k) & ~imm) | (w[0] & ~w[1]);
-      sacc[k][threadIdx.x] = (r0k & ~imm) | w[0];
+      sacc[4 + k][threadIdx.x] = lop3<0x54>(w[2] | r1k, w[0], w[1]);
+      sacc[k][threadIdx.x] = lop3<0xDC>(r0k, w[0], w[1]);
     }
     PE_FENCE();
   }

@@ -69,7 +69,7 @@ class Gen:
                 continue
             lo, hi = self.names[D], self.names[D | 1 << k]
             t = self.fresh()
-            self.lines.append("const uint32_t %s = ((%s >> %d) ^ %s) & 0x%08Xu; %s ^= %s; %s ^= %s << %d;"
+            self.lines.append("const uint32_t %s = lop3_32<kAndXor>(%s >> %d, %s, 0x%08Xu); %s ^= %s; %s ^= %s << %d;"
                               % (t, lo, s, hi, m, hi, t, lo, t, s))
             self.cost += 5
 
@@ -89,8 +89,8 @@ class Gen:
         for D in range(8):
             x = self.names[D]
             t = self.fresh()
-            self.lines.append("const uint32_t %s = (%s ^ (%s >> %d)) & 0x%08Xu; %s ^= %s ^ (%s << %d);"
-                              % (t, x, x, s, m, x, t, t, s))
+            self.lines.append("const uint32_t %s = lop3_32<kAndXor>(%s, %s >> %d, 0x%08Xu); %s = lop3_32<kXor3>(%s, %s, %s << %d);"
+                              % (t, x, x, s, m, x, x, t, t, s))
             self.cost += 4
 
     # Y(z) = X(z + k e) for the coordinate held in index bits (lo, lo + 1), lo = 0 or 2
@@ -112,8 +112,8 @@ class Gen:
             width = 16
         for D in range(8):
             x = self.names[D]
-            self.lines.append("%s = ((%s >> %d) & 0x%08Xu) | ((%s << %d) & 0x%08Xu);"
-                              % (x, x, sh, m, x, width - sh, ~m & 0xFFFFFFFF))
+            self.lines.append("%s = lop3_32<kSel>(%s >> %d, %s << %d, 0x%08Xu);"
+                              % (x, x, sh, x, width - sh, m))
             self.cost += 3
 
     # any map new[pos] = st[f(pos)] with f moving only index bits 3-7: dword renames and

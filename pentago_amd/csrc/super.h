@@ -68,6 +68,35 @@ PE_HD uint64_t select64(uint64_t a, uint64_t b, uint32_t m) {
   return (a & ~m64) | (b & m64);
 #endif
 }
+// Any bitwise function of three words, given by its truth table TT (bit 4a + 2b + c of TT is
+// the result for input bits a, b, c): one v_bitop3_b32 per 32-bit half on the device.  On
+// gfx950 it issues at the rate of a two-input and/or/xor, twice the rate of v_or3_b32,
+// v_and_or_b32 and v_bfi_b32, which the compiler would otherwise pick for these forms.
+template <int TT>
+PE_HD uint32_t lop3_32(uint32_t a, uint32_t b, uint32_t c) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_bitop3_b32(a, b, c, TT);
+#else
+  uint32_t r = 0;
+  for (int i = 0; i < 8; i++)
+    if (TT >> i & 1) r |= ((i & 4) ? a : ~a) & ((i & 2) ? b : ~b) & ((i & 1) ? c : ~c);
+  return r;
+#endif
+}
+template <int TT>
+PE_HD uint64_t lop3(uint64_t a, uint64_t b, uint64_t c) {
+  return uint64_t(lop3_32<TT>(uint32_t(a), uint32_t(b), uint32_t(c))) |
+         uint64_t(lop3_32<TT>(uint32_t(a >> 32), uint32_t(b >> 32), uint32_t(c >> 32))) << 32;
+}
+enum : int {
+  kOr3 = 0xFE,      // a | b | c
+  kSel = 0xE4,      // c ? a : b
+  kOrAnd = 0xF8,    // a | (b & c)
+  kAndXor = 0x28,   // (a ^ b) & c
+  kXor3 = 0x96,     // a ^ b ^ c
+  kAndAnd = 0x80,   // a & b & c
+};
+
 // Byte gather from the 8-byte value (hi:lo): byte k of the result is byte sel_k of it
 // (0-3 from lo, 4-7 from hi) or zero (sel_k = 12); v_perm_b32 on the device.
 PE_HD uint32_t pe_perm(uint32_t hi, uint32_t lo, uint32_t sel) {
@@ -106,11 +135,11 @@ PE_HD S4 shift_coord_var(S4 a, int q, int k) {
   k &= 3;
   if (q == 0) {
     const uint64_t m = k == 0 ? ~uint64_t(0) : k == 1 ? 0x7777777777777777ull : k == 2 ? 0x3333333333333333ull : 0x1111111111111111ull;
-    for (int i = 0; i < 4; i++) a.w[i] = ((a.w[i] >> k) & m) | ((a.w[i] << (4 - k)) & ~m);
+    for (int i = 0; i < 4; i++) a.w[i] = lop3<kSel>(a.w[i] >> k, a.w[i] << (4 - k), m);
   } else if (q == 1) {
     const uint64_t m = k == 0 ? ~uint64_t(0) : k == 1 ? 0x0FFF0FFF0FFF0FFFull : k == 2 ? 0x00FF00FF00FF00FFull : 0x000F000F000F000Full;
     const int s = 4 * k;
-    for (int i = 0; i < 4; i++) a.w[i] = ((a.w[i] >> s) & m) | ((a.w[i] << (16 - s)) & ~m);
+    for (int i = 0; i < 4; i++) a.w[i] = lop3<kSel>(a.w[i] >> s, a.w[i] << (16 - s), m);
   } else if (q == 2) {
     const int s = 16 * k;
     for (int i = 0; i < 4; i++) a.w[i] = (a.w[i] >> s) | (a.w[i] << ((64 - s) & 63));
@@ -213,12 +242,21 @@ PE_HD S4 transform_super(int g, int y, S4 c) { return rearrange(shift_local(c, y
 
 // rmax(f)(a) = OR over the 8 single-quadrant rotations b of f(a+b)
 // (pentago/base/superscore.h:245-306).
+// Per word: the +-1 rotations of r0 (inside nibbles) and r1 (nibbles inside 16-bit chunks) as
+// masked selects; those of r2 (16-bit chunks c0..c3) OR to (c1|c3, c0|c2) in both halves,
+// a 16-bit rotation of the OR of the halves; those of r3 are the neighbouring words.
 PE_HD S4 rmax(S4 a) {
   S4 r;
   for (int i = 0; i < 4; i++) {
     const uint64_t x = a.w[i];
-    r.w[i] = shift0_word(x, 1) | shift0_word(x, 3) | shift1_word(x, 1) | shift1_word(x, 3) |
-             shift2_word(x, 1) | shift2_word(x, 3) | a.w[(i + 1) & 3] | a.w[(i + 3) & 3];
+    const uint64_t s0 = lop3<kSel>(x >> 1, x << 3, 0x7777777777777777ull) |
+                        lop3<kSel>(x >> 3, x << 1, 0x1111111111111111ull);
+    const uint64_t s1 = lop3<kSel>(x >> 4, x << 12, 0x0FFF0FFF0FFF0FFFull) |
+                        lop3<kSel>(x >> 12, x << 4, 0x000F000F000F000Full);
+    const uint32_t h = uint32_t(x) | uint32_t(x >> 32);
+    const uint32_t h2 = h >> 16 | h << 16;
+    const uint64_t s2 = uint64_t(h2) | uint64_t(h2) << 32;
+    r.w[i] = lop3<kOr3>(lop3<kOr3>(s0, s1, s2), a.w[(i + 1) & 3], a.w[(i + 3) & 3]);
   }
   return r;
 }
@@ -226,20 +264,21 @@ PE_HD S4 rmax(S4 a) {
 // ---------------- super_wins (pentago/base/superscore.cc:38-96) ----------------
 // Table: per (quadrant, state) five supers (horizontal, vertical, diagonal lo, hi, assist).
 // OR-reductions over one coordinate axis:
+// (each stage ORs in the partner bit at distance s: mask ? x >> s : x << s)
 PE_HD uint64_t red0(uint64_t x) {
-  x |= ((x >> 1) & 0x5555555555555555ull) | ((x << 1) & 0xAAAAAAAAAAAAAAAAull);
-  x |= ((x >> 2) & 0x3333333333333333ull) | ((x << 2) & 0xCCCCCCCCCCCCCCCCull);
+  x |= lop3<kSel>(x >> 1, x << 1, 0x5555555555555555ull);
+  x |= lop3<kSel>(x >> 2, x << 2, 0x3333333333333333ull);
   return x;
 }
 PE_HD uint64_t red1(uint64_t x) {
-  x |= ((x >> 4) & 0x0F0F0F0F0F0F0F0Full) | ((x << 4) & 0xF0F0F0F0F0F0F0F0ull);
-  x |= ((x >> 8) & 0x00FF00FF00FF00FFull) | ((x << 8) & 0xFF00FF00FF00FF00ull);
+  x |= lop3<kSel>(x >> 4, x << 4, 0x0F0F0F0F0F0F0F0Full);
+  x |= lop3<kSel>(x >> 8, x << 8, 0x00FF00FF00FF00FFull);
   return x;
 }
 PE_HD uint64_t red2(uint64_t x) {
-  x |= (x >> 16) | (x << 48);
-  x |= (x >> 32) | (x << 32);
-  return x;
+  x = lop3<kOr3>(x, x >> 16, x << 48);
+  const uint64_t h = (x | (x >> 32)) & 0xFFFFFFFFull;
+  return h | h << 32;
 }
 PE_HD S4 reduce0(S4 a) { for (int i = 0; i < 4; i++) a.w[i] = red0(a.w[i]); return a; }
 PE_HD S4 reduce1(S4 a) { for (int i = 0; i < 4; i++) a.w[i] = red1(a.w[i]); return a; }
