@@ -26,17 +26,16 @@ __device__ __forceinline__ int dchoose(const int32_t* C, int n, int k) {
 
 // k-subset of [0,n) with the given index, 5 bits per entry, large-entry-major order
 // (pentago/mid/subsets.h:33-48)
+// Entry i is the largest j with choose(j, i + 1) <= index (the entries below it count
+// sum_{j' < j} choose(j', i) = choose(j, i + 1) subsets): counted over j' in (i, n), so the
+// table reads do not wait on each other.
 __device__ inline uint64_t dsubset(const int32_t* C, int n, int k, int index) {
   uint64_t set = 0;
   for (int i = k - 1; i >= 0; i--) {
-    int j = i, base = 0;
-    for (; j < n; j++) {
-      const int next = base + dchoose(C, j, i);
-      if (index < next) break;
-      base = next;
-    }
+    int j = i;
+    for (int jj = i + 1; jj < n; jj++) j += C[jj * 32 + i + 1] <= index;
     set |= uint64_t(j) << (5 * i);
-    index -= base;
+    index -= C[j * 32 + i + 1];   // choose(i, i + 1) = 0
   }
   return set;
 }
@@ -53,11 +52,11 @@ __device__ __forceinline__ int level_of(const int32_t* off, int spots, int64_t i
   return n;
 }
 
-// One table entry: index over [set0 | wins1 | cs1ps] of one board.
-__device__ inline void mid_prep_entry(const MidBoardDev& B, int64_t idx, const DevTables& T) {
+// One table entry: index over [set0 | wins1 | cs1ps] of one board.  C: the binomial table
+// (T.choose or a copy of it in LDS).
+__device__ inline void mid_prep_entry(const MidBoardDev& B, int64_t idx, const DevTables& T, const int32_t* C) {
   const MidInfo& I = B.I;
   const int spots = I.spots;
-  const int32_t* C = T.choose;
   const int64_t n_set0 = I.sets0_off[spots + 1], n_wins1 = I.wins1_off[spots + 1];
   if (idx < n_set0) {
     // set0_info (internal.h:174-259)

@@ -31,13 +31,18 @@ __device__ __forceinline__ int find_board(const int64_t* prefix, int nb, int64_t
   return lo;
 }
 
+// The binomial table (4 KB) is staged in LDS first: the subset unranking and the offset
+// loops below read it in chains of dependent lookups.
 __global__ __launch_bounds__(256) void mid_prep_kernel(const MidBoardDev* __restrict__ boards,
                                                        const int64_t* __restrict__ prefix, int nb,
                                                        int64_t total, DevTables T) {
+  __shared__ int32_t sC[32 * 32];
+  for (int i = threadIdx.x; i < 32 * 32; i += blockDim.x) sC[i] = T.choose[i];
+  __syncthreads();
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= total) return;
   const int b = find_board(prefix, nb, t);
-  mid_prep_entry(boards[b], t - prefix[b], T);
+  mid_prep_entry(boards[b], t - prefix[b], T, sC);
 }
 
 // Level launches of one board are spread over the 8 XCDs (workgroup i runs on XCD i % 8) by
