@@ -327,7 +327,7 @@ def line_dropin_bench(P, hip, n_lines=256, nthreads=8):
     # the first line of each (section, dimension) builds that pair's cached tables
     for d in range(4):
         pl, din, dout = jobs[d]
-        P._check(P.lib().pe_end_schedule_line(C.byref(pl), din.ptr, dout.ptr, streams[0].ptr, None, None),
+        P._check(P.lib().pe_end_schedule_line(C.byref(pl), din.ptr, dout.ptr, streams[0].ptr, P.WAKEUP(), None),
                  "pe_end_schedule_line")
     streams[0].synchronize()
     call_s = [0.0] * nthreads
@@ -644,11 +644,12 @@ def mid_bench(P, hip, n_boards=4096, seed=2, tune=True, plain=True):
         lat.append(1e3 * (time.perf_counter() - t0))
     out["single_26_stones_ms"] = float(np.median(lat))
     boards = seeded_boards(n_boards, seed, 18, 30)
-    sample = boards[:512]
     if tune:
+        # the autotuner times each candidate (one warm run) on the batch itself: the best split
+        # between the small-board kernel and the large boards' launches depends on the mix
         t0 = time.time()
-        tuned = P.mid_autotune(sample)
-        out["autotune"] = dict(tuned, seconds=time.time() - t0, boards=len(sample))
+        tuned = P.mid_autotune(boards)
+        out["autotune"] = dict(tuned, seconds=time.time() - t0, boards=len(boards))
         log("midsolve autotune: %s" % out["autotune"])
     t0 = time.time()
     plan = P.MidPlan(boards)

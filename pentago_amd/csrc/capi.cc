@@ -721,69 +721,68 @@ pe_status pe_mid_autotune(int n, const pe_high_board* boards, int64_t workspace_
   int dev = 0, cus = 0;
   PE_HIP(hipGetDevice(&dev), "hipGetDevice");
   PE_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute(CUs)");
-  const int32_t smalls[] = {12, 14, 16};
-  const int32_t threads[] = {256, 512, 1024};
   hipStream_t s = nullptr;
   PE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
   hipEvent_t e0 = nullptr, e1 = nullptr;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
-  float best_ms = -1;
-  pe_mid_config chosen{};
-  for (int32_t sm : smalls)
-    for (int32_t th : threads) {
-      pe_mid_config c{};
-      c.workspace_budget = workspace_budget;
-      c.small_spots = sm;
-      c.threads = th;
-      c.workgroups = cus * mid_board_blocks_per_cu(th);
-      MidConfig cfg;
-      st = mid_resolve_config(&c, &cfg);
-      if (st != PE_OK) break;
-      MidPlan plan;
-      st = mid_plan_create(infos, cfg, &plan);
-      float ms = -1;
-      if (st == PE_OK) st = mid_plan_run(plan, s);   // warm-up
-      for (int rep = 0; rep < 2 && st == PE_OK; rep++) {   // the faster of two timed runs
-        float t = 0;
-        if (hipEventRecord(e0, s) == hipSuccess) st = mid_plan_run(plan, s);
-        if (st == PE_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                            hipEventElapsedTime(&t, e0, e1) != hipSuccess))
-          st = fail(PE_ERR_DEVICE, "autotune timing failed");
-        if (st == PE_OK && (ms < 0 || t < ms)) ms = t;
-      }
-      (void)hipStreamSynchronize(s);
-      mid_plan_destroy(&plan);
-      if (st != PE_OK) break;
-      if (best_ms < 0 || ms < best_ms) {
-        best_ms = ms;
-        chosen = c;
-        chosen.streams = cfg.streams;
-      }
-    }
-  // the chosen configuration replayed as a captured graph (launch overhead of the large
-  // boards' level launches) against eager launches
-  chosen.use_graph = -1;
-  if (st == PE_OK) {
-    pe_mid_config c = chosen;
-    c.use_graph = 1;
+  // one untimed and one timed run of the whole batch per candidate (a new plan's first run
+  // touches freshly allocated workspace, and a graph candidate captures in it), so the boards
+  // handed in can be the real batch: the best split between the persistent small-board
+  // kernel and the large boards' level launches depends on how many boards of each kind
+  // there are
+  auto trial = [&](const pe_mid_config& c, float* ms) -> pe_status {
     MidConfig cfg;
-    st = mid_resolve_config(&c, &cfg);
+    pe_status r = mid_resolve_config(&c, &cfg);
+    if (r != PE_OK) return r;
     MidPlan plan;
-    if (st == PE_OK) st = mid_plan_create(infos, cfg, &plan);
-    float ms = -1;
-    if (st == PE_OK) st = mid_plan_run(plan, s);   // warm-up (captures the graph)
-    for (int rep = 0; rep < 2 && st == PE_OK; rep++) {
-      float t = 0;
-      if (hipEventRecord(e0, s) == hipSuccess) st = mid_plan_run(plan, s);
-      if (st == PE_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                          hipEventElapsedTime(&t, e0, e1) != hipSuccess))
-        st = fail(PE_ERR_DEVICE, "autotune timing failed");
-      if (st == PE_OK && (ms < 0 || t < ms)) ms = t;
-    }
+    r = mid_plan_create(infos, cfg, &plan);
+    if (r == PE_OK) r = mid_plan_run(plan, s);   // untimed
+    float t = 0;
+    if (r == PE_OK && hipEventRecord(e0, s) == hipSuccess) r = mid_plan_run(plan, s);
+    if (r == PE_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                       hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+      r = fail(PE_ERR_DEVICE, "autotune timing failed");
     (void)hipStreamSynchronize(s);
     mid_plan_destroy(&plan);
-    if (st == PE_OK && ms < best_ms) chosen.use_graph = 1;
+    *ms = t;
+    return r;
+  };
+  auto config = [&](int32_t sm, int32_t th, int32_t graph) {
+    pe_mid_config c{};
+    c.workspace_budget = workspace_budget;
+    c.small_spots = sm;
+    c.threads = th;
+    c.workgroups = cus * mid_board_blocks_per_cu(th);
+    c.use_graph = graph;
+    return c;
+  };
+  // stage 1: the small/large split at 512 threads; stage 2: threads per small-board
+  // workgroup at that split; stage 3: the winner replayed as a captured graph
+  float best_ms = -1;
+  pe_mid_config chosen{};
+  for (int32_t sm : {12, 14, 16}) {
+    float ms = 0;
+    const pe_mid_config c = config(sm, 512, -1);
+    if ((st = trial(c, &ms)) != PE_OK) break;
+    if (best_ms < 0 || ms < best_ms) { best_ms = ms; chosen = c; }
+  }
+  if (st == PE_OK)
+    for (int32_t th : {256, 1024}) {
+      float ms = 0;
+      const pe_mid_config c = config(chosen.small_spots, th, -1);
+      if ((st = trial(c, &ms)) != PE_OK) break;
+      if (ms < best_ms) { best_ms = ms; chosen = c; }
+    }
+  if (st == PE_OK) {
+    float ms = 0;
+    pe_mid_config c = chosen;
+    c.use_graph = 1;
+    if ((st = trial(c, &ms)) == PE_OK && ms < best_ms) chosen.use_graph = 1;
+  }
+  if (st == PE_OK) {
+    MidConfig cfg;
+    if ((st = mid_resolve_config(&chosen, &cfg)) == PE_OK) chosen.streams = cfg.streams;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
