@@ -20,10 +20,6 @@ struct MidBoardDev {
   pe_halfsupers* results;   // 19 entries
 };
 
-__device__ __forceinline__ int dchoose(const int32_t* C, int n, int k) {
-  return (n < 0 || k < 0 || k > n) ? 0 : C[n * 32 + k];
-}
-
 // k-subset of [0,n) with the given index, 5 bits per entry, large-entry-major order
 // (pentago/mid/subsets.h:33-48)
 // Entry i is the largest j with choose(j, i + 1) <= index (the entries below it count
@@ -80,21 +76,35 @@ __device__ inline void mid_prep_entry(const MidBoardDev& B, int64_t idx, const D
         k++;
       }
     auto empty1 = [&](int q) { return int((q < 12 ? e1[0] >> (5 * q) : e1[1] >> (5 * (q - 12))) & 0x1f); };
-    for (int i = 0; i < spots - k0; i++) {
-      const int j = empty1(i) - i;
-      uint16_t c = uint16_t(dchoose(C, empty1(i), j + 1));
-      for (int a = 0; a < k0; a++) c = uint16_t(c + dchoose(C, int(set0 >> (5 * a) & 0x1f), a + (a >= j) + 1));
-      out.child_s0s[i] = c;
-    }
-    for (int a = 0; a < k1; a++)
-      for (int q = 0; q < spots - k0; q++) {
-        uint16_t offset = uint16_t(a ? 0 : s0);
-        for (int i = empty1(q) - q; i < k0; i++) {
-          const int v = int(set0 >> (5 * i) & 0x1f);
-          if (v > a) offset = uint16_t(offset - dchoose(C, v - a - 1, i));
+    auto v = [&](int a) { return int(set0 >> (5 * a) & 0x1f); };
+    // child_s0s[i] = choose(empty1(i), j + 1) + sum_{a < j} choose(v_a, a + 1)
+    //              + sum_{a >= j} choose(v_a, a + 2),  j = empty1(i) - i (non-decreasing in i),
+    // with the two sums carried along as j grows (mod 2^16, as the term-by-term form)
+    {
+      int P = 0, S = 0, jc = 0;
+      for (int a = 0; a < k0; a++) S += C[v(a) * 32 + a + 2];
+      for (int i = 0; i < spots - k0; i++) {
+        const int j = empty1(i) - i;
+        for (; jc < j; jc++) {
+          P += C[v(jc) * 32 + jc + 1];
+          S -= C[v(jc) * 32 + jc + 2];
         }
-        out.offset0[a * (spots - k0) + q] = offset;
+        out.child_s0s[i] = uint16_t(C[empty1(i) * 32 + j + 1] + P + S);
       }
+    }
+    // offset0[a][q] = (a ? 0 : s0) - sum_{i >= empty1(q) - q, v_i > a} choose(v_i - a - 1, i):
+    // the suffix sum shrinks as q grows
+    for (int a = 0; a < k1; a++) {
+      int cur = 0;
+      for (int i = 0; i < k0; i++)
+        if (v(i) > a) cur += C[(v(i) - a - 1) * 32 + i];
+      int i = 0;
+      for (int q = 0; q < spots - k0; q++) {
+        for (const int t = empty1(q) - q; i < t; i++)
+          if (v(i) > a) cur -= C[(v(i) - a - 1) * 32 + i];
+        out.offset0[a * (spots - k0) + q] = uint16_t((a ? 0 : s0) - cur);
+      }
+    }
   } else if (idx < n_set0 + n_wins1) {
     // wins1 (internal.h:122-131)
     const int64_t e = idx - n_set0;
@@ -122,13 +132,16 @@ __device__ inline void mid_prep_entry(const MidBoardDev& B, int64_t idx, const D
     for (int a = 0; a < k1; a++) bits |= 1u << (set1p >> (5 * a) & 0x1f);
     B.set1p_bits[e] = bits;
     uint16_t* row = B.cs1ps + I.cs1ps_off[n] + int64_t(s1p) * (spots - k0);
+    // row[j] = s1p - sum_{a: v_a > j} choose(v_a - 1, a); the members are increasing, so the
+    // sum drops them one by one as j grows
+    auto v = [&](int a) { return int(set1p >> (5 * a) & 0x1f); };
+    int cur = 0, p = 0;
+    for (int a = 0; a < k1; a++)
+      if (v(a) > 0) cur += C[(v(a) - 1) * 32 + a];
+    while (p < k1 && v(p) <= 0) p++;
     for (int j = 0; j < spots - k0; j++) {
-      uint16_t c = uint16_t(s1p);
-      for (int a = 0; a < k1; a++) {
-        const int v = int(set1p >> (5 * a) & 0x1f);
-        if (j < v) c = uint16_t(c - dchoose(C, v - 1, a));
-      }
-      row[j] = c;
+      for (; p < k1 && v(p) <= j; p++) cur -= C[(v(p) - 1) * 32 + p];
+      row[j] = uint16_t(s1p - cur);
     }
   }
 }
