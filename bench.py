@@ -669,6 +669,8 @@ def mid_bench(P, hip, n_boards=4096, seed=2, tune=True, plain=True):
         ms.append(ev0.elapsed_ms(ev1))
     res2 = plan.results(stream.ptr)
     plan.close()
+    # the first boards' results, for the oracle check in the CPU baseline leg (not printed)
+    out["_first_results"] = P.results_to_lists(boards[:48], res2[:48])
     batched_s = min(ms) / 1e3
     log("midsolve batched: %s ms" % ms)
     spots = np.array([board_spots(b) for b in boards])
@@ -792,10 +794,11 @@ def serve_bench(P, rates=(200, 1000, 5000), seed=3, seconds=2.0):
     return out
 
 
-def mid_cpu_baseline(n=48, seed=2):
+def mid_cpu_baseline(n=48, seed=2, expect=None, batch=4096):
     """The oracle's midsolve (two passes per board, one board per thread) on a sample of the
     batched workload (all-core), single-thread on the 26-stone board of the single-board
-    configuration and on 24..30-stone boards of the batch."""
+    configuration and on 24..30-stone boards of the batch.  `expect`: the batched plan's
+    results for the same first boards, compared entry by entry with the oracle's."""
     from pentago_amd.workloads import seeded_boards
     O, L, build = oracle_for_timing()
     threads = baseline_threads()
@@ -803,10 +806,14 @@ def mid_cpu_baseline(n=48, seed=2):
     def roots(bs):
         return [O.High((O.C.c_uint64 * 2)(b.side[0], b.side[1]), b.ply) for b in bs]
 
-    boards = seeded_boards(4096, seed, 18, 30)[:n]
+    boards = seeded_boards(batch, seed, 18, 30)[:n]
     t0 = time.time()
-    O.midsolve_batch(roots(boards), threads, L=L)
+    got = O.midsolve_batch(roots(boards), threads, L=L)
     dt = time.time() - t0
+    oracle_check = None
+    if expect is not None:
+        bad = sum(1 for e, g, b in zip(expect, got, boards) if list(e) != list(g[:len(e)]))
+        oracle_check = {"boards": min(len(expect), n), "mismatches": bad}
     one = seeded_boards(1, 26, 26, 26)[0]
     r1 = roots([one])
     t0 = time.time()
@@ -824,7 +831,7 @@ def mid_cpu_baseline(n=48, seed=2):
                       "threads" % (n, dt, threads),
             "single_26_stones": {"boards_per_s": reps / dt1, "ms": 1e3 * dt1 / reps, "cores": 1},
             "single_thread_24_30_stones": {"boards_per_s": len(small) / dts, "boards": len(small), "cores": 1},
-            "host": host_info(), "build": build}
+            "batched_vs_oracle": oracle_check, "host": host_info(), "build": build}
 
 
 def main():
@@ -977,6 +984,7 @@ def main():
         result["slices"] = slices_bench(P, hip, slice32=args.slice32)
     if not args.no_mid:
         result["midsolve"] = mid_bench(P, hip, args.mid_boards, tune=args.mid_tune, plain=args.mid_plain)
+        first = result["midsolve"].pop("_first_results")
     if not args.no_serve:
         result["serving"] = serve_bench(P)
     if not args.no_cpu and not args.no_end:
@@ -986,7 +994,7 @@ def main():
             result["cpu_baseline"] = {"error": str(e)}
     if not args.no_cpu and not args.no_mid:
         try:
-            result["midsolve"]["cpu_baseline"] = mid_cpu_baseline()
+            result["midsolve"]["cpu_baseline"] = mid_cpu_baseline(expect=first, batch=args.mid_boards)
         except Exception as e:
             result["midsolve"]["cpu_baseline"] = {"error": str(e)}
     print(json.dumps(result), flush=True)
