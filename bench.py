@@ -901,13 +901,16 @@ def main():
 
         stream = hip.Stream()
         launch = None
+        form = 0
         if args.mode == 0 and args.launch >= 0:
             plan.set_launch(args.launch)
+            form = args.launch
             launch = {"choice": "grid" if args.launch == 0 else "queue, %d workgroups per CU" % args.launch}
         elif args.mode == 0:
             # launch form: static grid against the persistent queue-fed grid, fastest kept
             t0 = time.time()
             forms, choice = plan.autotune(stream.ptr)
+            form = choice
             launch = {"choice": "grid" if choice == 0 else "queue, %d workgroups per CU" % choice,
                       "timed_ms": {("grid" if f == 0 else "queue_%d_per_cu" % f): ms for f, ms in forms.items()},
                       "seconds": time.time() - t0}
@@ -936,7 +939,9 @@ def main():
         # algorithmic bytes per step: box output once + every line's child nodes once
         alg_step = 64 * (n_out + child_nodes)
         achieved = (alg_step / launches) / (avg_launch_ms / 1e3) / 1e9
-        kernel = "fused_blocks_kernel" if args.mode == 0 else "end_lines_kernel"
+        # the dominant kernel: the grid form, the queue-fed form, or the line kernel
+        kernel = ("fused_blocks_kernel" if form == 0 else "fused_blocks_queue_kernel") if args.mode == 0 \
+            else "end_lines_kernel"
         traffic = None
         tag = "box%d" % k + ("_mixed%d" % k2 if k2 else "")
         tfile = os.path.join(ROOT, "profiles", "%s_%s_traffic.json" % (kernel, tag))
