@@ -107,10 +107,11 @@ constexpr int kThreads = 256;
 constexpr int kParts = 4096 / kThreads;
 constexpr int kChunk = 1 << 19;   // blocks per grid launch: 2^31 work-items
 // Occupancy: each node's running OR over its dimensions (and then its output words) lives in
-// LDS, so a wave needs ~74 VGPRs and five waves share a SIMD; children are loaded one move
-// at a time.  Measured on the headline launch against keeping them in registers (123 VGPRs,
-// four waves, moves loaded three at a time): 157.4 vs 168.4 ms; two moves at a time 162.5,
-// six waves 159.1.
+// LDS (16 KB per workgroup, 24 KB with the field entries), so a wave needs 66 VGPRs and six
+// waves share a SIMD (the LDS limit); children are loaded one move at a time.  Measured on
+// the headline launch against keeping them in registers (123 VGPRs, four waves, moves loaded
+// three at a time): 157.4 vs 168.4 ms; two moves at a time 162.5.  The launch bound asks for
+// at least five waves' worth of registers.
 constexpr int kWaves = 5;
 
 // One output block of 4096 nodes, computed by 16 workgroups (`part` = which 256 nodes).
