@@ -5,6 +5,7 @@ the bit manipulation is checked before it runs on the GPU.  No GPU needed."""
 import ctypes as C
 import os
 import subprocess
+import sys
 import tempfile
 
 import numpy as np
@@ -109,3 +110,12 @@ def test_split_and_half_rmax(hc):
         out = (C.c_uint64 * 2)()
         hc.hc_hrmax(arr(x, 2), out)
         assert val(out) == O.half_rmax(x)
+
+
+def test_generated_transforms_header_is_current():
+    """pentago_amd/csrc/transforms.h is the output of gen_transforms.py (whose plans are
+    simulated on all 256 bit positions before they are written)."""
+    csrc = os.path.join(HERE, "..", "pentago_amd", "csrc")
+    out = subprocess.check_output([sys.executable, os.path.join(csrc, "gen_transforms.py")], text=True)
+    with open(os.path.join(csrc, "transforms.h")) as f:
+        assert f.read() == out
