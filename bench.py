@@ -694,21 +694,31 @@ def mid_bench(P, hip, n_boards=4096, seed=2, tune=True, plain=True):
         # small boards alone (24..30 stones, 6..12 empty cells): launch overhead dominates the
         # plain path, the batched kernel solves them one workgroup per board
         small = seeded_boards(n_boards, seed + 1, 24, 30)
-        plan = P.MidPlan(small)
-        plan.run(stream.ptr)
-        sres = plan.results(stream.ptr)
-        ev0.record(stream)
-        plan.run(stream.ptr)
-        ev1.record(stream)
-        ev1.synchronize()
-        small_ms = ev0.elapsed_ms(ev1)
-        plan.close()
+        # the small-board kernel's workgroup size, picked on this batch (an explicit plan
+        # configuration: the device default stays the mix's)
+        small_ms, sres, small_threads = None, None, None
+        for th in (256, 512, 1024):
+            plan = P.MidPlan(small, small_spots=12, threads=th)
+            plan.run(stream.ptr)
+            res_th = plan.results(stream.ptr)
+            ev0.record(stream)
+            plan.run(stream.ptr)
+            ev1.record(stream)
+            ev1.synchronize()
+            t = ev0.elapsed_ms(ev1)
+            plan.close()
+            if sres is not None:
+                mism += int(np.sum(np.any(res_th != sres, axis=(1, 2))))
+            if small_ms is None or t < small_ms:
+                small_ms, small_threads = t, th
+            sres = res_th
         splain = np.zeros_like(sres)
         t0 = time.perf_counter()
         for i, b in enumerate(small):
             P.midsolve_internal_raw(b, splain[i])
         small_plain_s = time.perf_counter() - t0
         out["small_boards"] = {"workload": "%d seeded boards, 24..30 stones (6..12 empty cells)" % n_boards,
+                               "threads": small_threads,
                                "batched_boards_per_s": n_boards / (small_ms / 1e3), "batched_ms": small_ms,
                                "plain_boards_per_s": n_boards / small_plain_s,
                                "batched_over_plain": small_plain_s / (small_ms / 1e3),

@@ -338,7 +338,8 @@ pe_status pe_midsolve_batch(int n, const pe_high_board* boards, pe_halfsupers (*
 typedef struct {
   int64_t workspace_budget;   /* bytes held by one run (0: 60% of free HBM, <= 200 GB) */
   int32_t small_spots;        /* boards with at most this many empty cells: one workgroup each (0: 14) */
-  int32_t workgroups;         /* persistent workgroups for small boards (0: a full device) */
+  int32_t workgroups;         /* persistent workgroups for small boards (0: the tuned count when
+                                 threads is the tuned size, else a full device of `threads`) */
   int32_t threads;            /* threads per small-board workgroup: 256, 512, 1024 (0: 512) */
   int32_t streams;            /* streams for large boards (0: 2; negative: all on the run stream) */
   int32_t use_graph;          /* > 0: pe_mid_plan_run replays a captured hipGraph; < 0: eager

@@ -583,7 +583,10 @@ pe_status mid_resolve_config(const pe_mid_config* in, MidConfig* out) {
   if (c.small_spots <= 0) c.small_spots = t.set ? t.small_spots : 14;
   if (c.threads <= 0) c.threads = t.set ? t.threads : 512;
   if (c.threads != 256 && c.threads != 512 && c.threads != 1024) return fail(PE_ERR_INVALID, "threads must be 256, 512 or 1024");
-  if (c.workgroups <= 0) c.workgroups = t.set ? t.workgroups : cus * mid_board_blocks_per_cu(c.threads);
+  // the tuned workgroup count belongs to the tuned workgroup size; another size given by the
+  // caller gets a full device of its own
+  if (c.workgroups <= 0)
+    c.workgroups = t.set && c.threads == t.threads ? t.workgroups : cus * mid_board_blocks_per_cu(c.threads);
   if (c.streams < 0) c.streams = 0;
   else if (c.streams == 0) c.streams = 2;
   out->budget = c.workspace_budget;
