@@ -379,12 +379,15 @@ def section_by_passes(P, hip, counts, samples, stream, seed_base=0x5EC30000, gro
     dimensions' child sections (seeded on the device, not timed; a child shared by two
     dimensions is held once).  By default the open dimensions are grouped greedily, in order,
     into as few passes as fit beside the output in `headroom` of the free device memory.  A
-    pass is one fused launch with only its dimensions open (pe_end_slabs_plan_create_dims),
-    the later passes OR-accumulating into the output (the block store's merge,
-    pentago/end/block_store.cc:173-229); each pass runs twice (OR-accumulating twice leaves the
-    same output) and the second run is timed.  For every sampled block the oracle's line
-    through it is computed from child data regenerated on the host from the seed.
-    Returns (output buffer, block counts, passes, {block: expected nodes})."""
+    pass is one fused launch with only its dimensions open (pe_end_slabs_plan_create_dims):
+    the first pass stores its children's OR in the output slots as a partial, middle passes
+    add to it, and the last finishes it (the win patterns and the result, once per node), the
+    same output as the block store's merge of per-dimension results
+    (pentago/end/block_store.cc:173-229).  The sequence of passes runs twice (a finishing
+    pass reads the partial, so a pass alone cannot run again) and the second is timed.  For
+    every sampled block the oracle's line through it is computed from child data regenerated
+    on the host from the seed.  Returns (output buffer, block counts, passes,
+    {block: expected nodes})."""
     from oracle import oracle as O
     from tests.seeded import seeded_nodes
     shape = P.section_shape(counts)
@@ -408,34 +411,41 @@ def section_by_passes(P, hip, counts, samples, stream, seed_base=0x5EC30000, gro
     out = hip.Buffer(out_bytes)
     out_slab = P.Slab.of(counts, [0] * 4, nb, out.ptr)
     ev0, ev1 = hip.Event(), hip.Event()
-    expect = {b: None for b in samples}
     passes = []
-    for group in groups:
-        kids = {}
-        for c in {child[d] for d in group}:
-            kids[c] = hip.Buffer(cbytes[c])
-            P.fill_seeded(kids[c].ptr, 4096 * int(np.prod(cnb[c])), seed_base + sum(v * 10 ** i for i, v in enumerate(sum(c, ()))))
-        t0 = time.time()
-        plan = P.Plan.slab_dims(out_slab, [P.Slab.of(c, [0] * 4, cnb[c], b.ptr) for c, b in kids.items()],
-                                sum(1 << d for d in group), accumulate=bool(passes))
-        build_s = time.time() - t0
-        hip.synchronize()
-        ms = []
-        for _ in range(2):
+    for rep in range(2):
+        for i, group in enumerate(groups):
+            # 0 one pass; else 2 store the partial, 4 add to it, 3 finish it
+            mode = 0 if len(groups) == 1 else 2 if i == 0 else 3 if i == len(groups) - 1 else 4
+            kids = {}
+            for c in {child[d] for d in group}:
+                kids[c] = hip.Buffer(cbytes[c])
+                P.fill_seeded(kids[c].ptr, 4096 * int(np.prod(cnb[c])), seed_base + sum(v * 10 ** j for j, v in enumerate(sum(c, ()))))
+            t0 = time.time()
+            plan = P.Plan.slab_dims(out_slab, [P.Slab.of(c, [0] * 4, cnb[c], b.ptr) for c, b in kids.items()],
+                                    sum(1 << d for d in group), accumulate=mode)
+            build_s = time.time() - t0
+            hip.synchronize()
             ev0.record(stream)
             plan.run(stream.ptr)
             ev1.record(stream)
             ev1.synchronize()
-            ms.append(ev0.elapsed_ms(ev1))
-        plan.close()
-        # algorithmic bytes: every child node of the pass's lines once, the output written once
-        # (and read once more by every pass after the first)
-        child_nodes = sum(int(np.prod(P.section_shape(child[d]))) for d in group)
-        passes.append({"dimensions": group, "children": sorted("".join("%d%d" % x for x in c) for c in kids),
-                       "child_GB": sum(cbytes[c] for c in kids) / 1e9, "ms": ms[1], "first_run_ms": ms[0],
-                       "plan_build_s": build_s,
-                       "algorithmic_bytes": 64 * (child_nodes + (2 if passes else 1) * n_out)})
-        log("section %s, dimensions %s: %s" % ("".join("%d%d" % c for c in counts), group, passes[-1]))
+            ms = ev0.elapsed_ms(ev1)
+            plan.close()
+            if rep == 0:
+                # algorithmic bytes: every child node of the pass's lines once, the output
+                # slots written once (and read once more by every pass after the first)
+                child_nodes = sum(int(np.prod(P.section_shape(child[d]))) for d in group)
+                passes.append({"dimensions": group, "mode": mode,
+                               "children": sorted("".join("%d%d" % x for x in c) for c in kids),
+                               "child_GB": sum(cbytes[c] for c in kids) / 1e9, "first_run_ms": ms,
+                               "plan_build_s": build_s,
+                               "algorithmic_bytes": 64 * (child_nodes + (2 if i else 1) * n_out)})
+            else:
+                passes[i]["ms"] = ms
+                log("section %s, dimensions %s: %s" % ("".join("%d%d" % c for c in counts), group, passes[i]))
+            kids = None
+    expect = {b: None for b in samples}
+    for group in groups:
         for d in group:
             c = child[d]
             seed = seed_base + sum(v * 10 ** i for i, v in enumerate(sum(c, ())))
@@ -456,7 +466,6 @@ def section_by_passes(P, hip, counts, samples, stream, seed_base=0x5EC30000, gro
                 bn = int(np.prod([min(8, shape[a] - 8 * blk[a]) for a in range(4)]))
                 part = line_out[8 * oi.block_stride * blk[d]:8 * oi.block_stride * blk[d] + 8 * bn]
                 expect[blk] = part if expect[blk] is None else expect[blk] | part
-        kids = None
     return out, nb, passes, expect
 
 

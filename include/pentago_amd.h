@@ -172,11 +172,18 @@ pe_status pe_end_slabs_plan_create(int noutputs, const pe_slab* outputs, int nch
 
 /* The fused plan of pe_end_slabs_plan_create (mode 0) for a subset of the open dimensions:
  * bit d of `dims` set = dimension d computed (its moves and its child lines), the others
- * skipped, so only the computed dimensions' child slabs are needed; with `accumulate` the
- * result is ORed into the output (the block-store merge, pentago/end/block_store.cc:173-229),
- * so runs over disjoint subsets covering every open dimension give the full result.  For
- * sections too large to hold with all their children at once.  Sections with at most 34
- * stones. */
+ * skipped, so only the computed dimensions' child slabs are needed.  For sections too large
+ * to hold with all their children at once; sections with at most 34 stones.  `accumulate`:
+ *   0  store the result of the computed dimensions;
+ *   1  OR it into the output (the block-store merge, pentago/end/block_store.cc:173-229), so
+ *      runs over disjoint subsets covering every open dimension give the full result;
+ *   2  store a partial instead: the computed dimensions' children only, transformed and ORed
+ *      (the output slots then hold an intermediate, not a result);
+ *   4  OR the computed dimensions' children into the partial in the output slots;
+ *   3  finish: the partial in the slots plus the computed dimensions' children, then the
+ *      result, stored.
+ * Runs 2, 4 ..., 3 over disjoint subsets give the same output as one plan over every
+ * dimension and evaluate the win-pattern groups once, in the finishing run. */
 pe_status pe_end_slabs_plan_create_dims(int noutputs, const pe_slab* outputs, int nchildren, const pe_slab* children,
                                         uint32_t dims, int accumulate, pe_end_plan** plan);
 

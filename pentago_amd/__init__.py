@@ -315,16 +315,17 @@ class Plan:
         return self
 
     @classmethod
-    def slab_dims(cls, output, children, dims, accumulate=False):
+    def slab_dims(cls, output, children, dims, accumulate=0):
         """Fused plan over the output slab(s) computing only the open dimensions in the bit
-        mask `dims` (pe_end_slabs_plan_create_dims), OR-accumulating when asked."""
+        mask `dims` (pe_end_slabs_plan_create_dims).  accumulate: 0 store, 1 OR into the
+        output, 2 store a partial (children only), 4 add to the partial, 3 finish from it."""
         self = cls.__new__(cls)
         outs = list(output) if isinstance(output, (list, tuple)) else [output]
         n = len(children)
         arr = (Slab * max(n, 1))(*children)
         oarr = (Slab * len(outs))(*outs)
         h = C.c_void_p()
-        _check(lib().pe_end_slabs_plan_create_dims(len(outs), oarr, n, arr, int(dims), int(bool(accumulate)),
+        _check(lib().pe_end_slabs_plan_create_dims(len(outs), oarr, n, arr, int(dims), int(accumulate),
                                                    C.byref(h)), "pe_end_slabs_plan_create_dims")
         self.handle = h
         self.nodes = lib().pe_end_plan_nodes(h)

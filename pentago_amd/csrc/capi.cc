@@ -368,6 +368,7 @@ pe_status pe_end_slabs_plan_create_dims(int noutputs, const pe_slab* outputs, in
   PE_GUARD_BEGIN
   if (noutputs <= 0 || !outputs || !out || nchildren < 0 || (nchildren && !children) || !(dims & 0xf) || dims > 0xf)
     return fail(PE_ERR_INVALID, "bad slab plan arguments");
+  if (accumulate < 0 || accumulate > 4) return fail(PE_ERR_INVALID, "accumulate must be 0..4");
   for (int k = 0; k < noutputs; k++) {
     if (!outputs[k].data) return fail(PE_ERR_INVALID, "output slab without memory");
     int sum = 0;
@@ -376,7 +377,7 @@ pe_status pe_end_slabs_plan_create_dims(int noutputs, const pe_slab* outputs, in
   }
   auto* plan = new pe_end_plan();
   std::string err;
-  const pe_status s = fused_plan_create(noutputs, outputs, nchildren, children, &plan->fused, &err, dims, accumulate ? 1 : 0);
+  const pe_status s = fused_plan_create(noutputs, outputs, nchildren, children, &plan->fused, &err, dims, accumulate);
   if (s != PE_OK) {
     fused_plan_destroy(&plan->fused);
     delete plan;

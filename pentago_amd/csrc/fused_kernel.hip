@@ -134,14 +134,21 @@ __device__ __forceinline__ void fused_block(const BlockDev& B, const SecDev& S, 
   const int bs0 = min(8, S.rmin_size[0] - 8 * bb[0]), bs1 = min(8, S.rmin_size[1] - 8 * bb[1]);
   const int bs2 = min(8, S.rmin_size[2] - 8 * bb[2]), bs3 = min(8, S.rmin_size[3] - 8 * bb[3]);
   // the block's field entries of quadrants 2 and 3 (eight each) staged in LDS
+  // B.accumulate: 0 store the result, 1 OR it into the output; partial passes (sections run
+  // in passes over subsets of their dimensions): 2 store the children's OR only, 4 add to
+  // the partial in the output slots, 3 finish from it (the result, stored)
+  const int mode = B.accumulate;
+  const bool partial_out = !S35 && (mode == 2 || mode == 4);
   __shared__ uint64_t sf[2][8][60];
-  for (int i = threadIdx.x; i < 2 * 8 * 60; i += kThreads) {
-    const int q = 2 + i / 480, r = (i / 60) % 8, w = i % 60;
-    const int bsq = q == 2 ? bs2 : bs3, bq = q == 2 ? bb[2] : bb[3];
-    const int64_t fo = q == 2 ? S.field_off[2] : S.field_off[3];
-    sf[q - 2][r][w] = r < bsq ? fields[4 * fo + 60 * int64_t(8 * bq + r) + w] : 0;
+  if (!partial_out) {   // uniform over the workgroup (one block)
+    for (int i = threadIdx.x; i < 2 * 8 * 60; i += kThreads) {
+      const int q = 2 + i / 480, r = (i / 60) % 8, w = i % 60;
+      const int bsq = q == 2 ? bs2 : bs3, bq = q == 2 ? bb[2] : bb[3];
+      const int64_t fo = q == 2 ? S.field_off[2] : S.field_off[3];
+      sf[q - 2][r][w] = r < bsq ? fields[4 * fo + 60 * int64_t(8 * bq + r) + w] : 0;
+    }
+    __syncthreads();
   }
-  __syncthreads();
   int loc[4] = {wave >> 3, wave & 7, lane >> 3, lane & 7};
   if (loc[0] >= bs0 || loc[1] >= bs1 || loc[2] >= bs2 || loc[3] >= bs3) return;
   const int p = ((loc[0] * bs1 + loc[1]) * bs2 + loc[2]) * bs3 + loc[3];
@@ -152,8 +159,18 @@ __device__ __forceinline__ void fused_block(const BlockDev& B, const SecDev& S, 
   // the node's running OR over the dimensions, then its output, in LDS: its own 8 words
   // ([word][thread], so a wave's accesses are consecutive 8-byte slots), never shared
   __shared__ uint64_t sacc[S35 ? 1 : 8][kThreads];
-  if (!S35)
-    for (int w = 0; w < 8; w++) sacc[w][threadIdx.x] = 0;
+  uint64_t* o = B.out + 8 * p;
+  if (!S35) {
+    if (mode >= 3) {   // the partial left in the output slots by an earlier pass
+      const S4 a = ld(o), b = ld(o + 4);
+      for (int w = 0; w < 4; w++) {
+        sacc[w][threadIdx.x] = a.w[w];
+        sacc[4 + w][threadIdx.x] = b.w[w];
+      }
+    } else {
+      for (int w = 0; w < 8; w++) sacc[w][threadIdx.x] = 0;
+    }
+  }
 #pragma unroll
   for (int d = 0; d < 4; d++) {
     if (S35 || bline[d] < 0) continue;
@@ -208,6 +225,16 @@ __device__ __forceinline__ void fused_block(const BlockDev& B, const SecDev& S, 
     for (int w = 0; w < 4; w++) sacc[w][threadIdx.x] |= x0.w[w];
 #pragma unroll
     for (int w = 0; w < 4; w++) sacc[4 + w][threadIdx.x] |= x1.w[w];
+  }
+  if (partial_out) {   // rmax(A) | rmax(B) = rmax(A | B): the finishing pass applies it once
+    S4 a, b;
+    for (int w = 0; w < 4; w++) {
+      a.w[w] = sacc[w][threadIdx.x];
+      b.w[w] = sacc[4 + w][threadIdx.x];
+    }
+    st(o, a);
+    st(o + 4, b);
+    return;
   }
   if (!S35) {   // rmax in place: words 0-3 from the children's [0] halves, 4-7 from [1]
 #pragma unroll
@@ -287,8 +314,7 @@ __device__ __forceinline__ void fused_block(const BlockDev& B, const SecDev& S, 
       o0.w[k] = sacc[4 + k][threadIdx.x];
       o1.w[k] = sacc[k][threadIdx.x];
     }
-  uint64_t* o = B.out + 8 * p;
-  if (B.accumulate) {   // OR-merge with other dimensions' lines (block_store.cc:173-229)
+  if (mode == 1) {   // OR-merge with other dimensions' lines (block_store.cc:173-229)
     o0 = s4_or(o0, ld(o));
     o1 = s4_or(o1, ld(o + 4));
   }
@@ -553,8 +579,9 @@ static void build_slab_body(const pe_slab& out, int nch, const pe_slab* ch, uint
     const int b = rmin_bucket(s.c[a][0], s.c[a][1]);
     sec.rmin_off[a] = Tb.rmin_offsets[b];
     sec.rmin_size[a] = Tb.rmin_offsets[b + 1] - Tb.rmin_offsets[b];
-    // a quadrant's moves count only when its dimension is computed
-    sec.field_off[a] = field_pool_off(a, sec.turn, (dims >> a) & 1, s.c[a][0], s.c[a][1]);
+    // a quadrant's moves count only when its dimension is computed, except in a finishing
+    // pass (accumulate 3), which evaluates the win patterns of every move for the whole node
+    sec.field_off[a] = field_pool_off(a, sec.turn, accumulate == 3 || ((dims >> a) & 1), s.c[a][0], s.c[a][1]);
     if (!(dims >> a & 1)) { sec.move_off[a] = -1; sec.nmoves[a] = 0; }
   }
   int b[4];
