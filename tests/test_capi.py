@@ -99,3 +99,15 @@ def test_count_many_arguments_checked_on_the_host():
         P.count_blocks_many([slab, slab], 1 << 20, None)
     with pytest.raises(P.PentagoError):
         P.count_blocks_many([slab], None, 1 << 20)
+
+
+def test_slab_dims_arguments_checked_on_the_host():
+    # the dimension mask, the accumulate mode (0-4) and the stone count are rejected before
+    # any device call; no slab memory is touched
+    slab = P.Slab.of(((3, 4), (4, 4), (4, 4), (4, 3)), [0] * 4, [1] * 4, 1 << 20)
+    for dims, acc in ((0, 0), (0x10, 0), (1, -1), (1, 5)):
+        with pytest.raises(P.PentagoError):
+            P.Plan.slab_dims(slab, [], dims, accumulate=acc)
+    full = P.Slab.of(((4, 4), (4, 4), (4, 4), (4, 3)), [0] * 4, [1] * 4, 1 << 20)
+    with pytest.raises(P.PentagoError):
+        P.Plan.slab_dims(full, [], 1, accumulate=3)
