@@ -383,8 +383,10 @@ def section_by_passes(P, hip, counts, samples, stream, seed_base=0x5EC30000, gro
     the first pass stores its children's OR in the output slots as a partial, middle passes
     add to it, and the last finishes it (the win patterns and the result, once per node), the
     same output as the block store's merge of per-dimension results
-    (pentago/end/block_store.cc:173-229).  The sequence of passes runs twice (a finishing
-    pass reads the partial, so a pass alone cannot run again) and the second is timed.  For
+    (pentago/end/block_store.cc:173-229).  The sequence of passes runs three times (a
+    finishing pass reads the partial, so a pass alone cannot run again): with every pass on
+    the static grid, then on the queue-fed form, then each pass on the faster of the two for
+    it, which is the timed `ms`.  For
     every sampled block the oracle's line through it is computed from child data regenerated
     on the host from the seed.  Returns (output buffer, block counts, passes,
     {block: expected nodes})."""
@@ -412,10 +414,15 @@ def section_by_passes(P, hip, counts, samples, stream, seed_base=0x5EC30000, gro
     out_slab = P.Slab.of(counts, [0] * 4, nb, out.ptr)
     ev0, ev1 = hip.Event(), hip.Event()
     passes = []
-    for rep in range(2):
+    queue = 6   # workgroups per CU of the queue-fed form
+    for rep in range(3):
         for i, group in enumerate(groups):
             # 0 one pass; else 2 store the partial, 4 add to it, 3 finish it
             mode = 0 if len(groups) == 1 else 2 if i == 0 else 3 if i == len(groups) - 1 else 4
+            if rep < 2:
+                form = (0, queue)[rep]
+            else:
+                form = 0 if passes[i]["grid_ms"] <= passes[i]["queue_ms"] else queue
             kids = {}
             for c in {child[d] for d in group}:
                 kids[c] = hip.Buffer(cbytes[c])
@@ -423,6 +430,7 @@ def section_by_passes(P, hip, counts, samples, stream, seed_base=0x5EC30000, gro
             t0 = time.time()
             plan = P.Plan.slab_dims(out_slab, [P.Slab.of(c, [0] * 4, cnb[c], b.ptr) for c, b in kids.items()],
                                     sum(1 << d for d in group), accumulate=mode)
+            plan.set_launch(form)
             build_s = time.time() - t0
             hip.synchronize()
             ev0.record(stream)
@@ -437,11 +445,14 @@ def section_by_passes(P, hip, counts, samples, stream, seed_base=0x5EC30000, gro
                 child_nodes = sum(int(np.prod(P.section_shape(child[d]))) for d in group)
                 passes.append({"dimensions": group, "mode": mode,
                                "children": sorted("".join("%d%d" % x for x in c) for c in kids),
-                               "child_GB": sum(cbytes[c] for c in kids) / 1e9, "first_run_ms": ms,
+                               "child_GB": sum(cbytes[c] for c in kids) / 1e9, "grid_ms": ms,
                                "plan_build_s": build_s,
                                "algorithmic_bytes": 64 * (child_nodes + (2 if i else 1) * n_out)})
+            elif rep == 1:
+                passes[i]["queue_ms"] = ms
             else:
                 passes[i]["ms"] = ms
+                passes[i]["launch"] = "grid" if form == 0 else "queue, %d workgroups per CU" % form
                 log("section %s, dimensions %s: %s" % ("".join("%d%d" % c for c in counts), group, passes[i]))
             kids = None
     expect = {b: None for b in samples}

@@ -133,7 +133,8 @@ pe_status pe_end_plan_set_launch(pe_end_plan* plan, int queue_wgs_per_cu);
 /* Time the launch forms {grid, 2, 3, 4, 5, 6 workgroups per CU} on this plan (warm-up, then the
  * faster of two runs each, on `stream`; the plan's output is recomputed, identical each
  * time), keep the fastest for the plan and as the current device's default for plans created
- * later.  No counterpart in the reference. */
+ * later.  PE_ERR_INVALID for a finishing pass (pe_end_slabs_plan_create_dims with
+ * accumulate 3), whose run is not repeatable.  No counterpart in the reference. */
 typedef struct {
   int32_t queue_wgs_per_cu;   /* the choice (0: grid) */
   int32_t candidates;
@@ -183,7 +184,9 @@ pe_status pe_end_slabs_plan_create(int noutputs, const pe_slab* outputs, int nch
  *   3  finish: the partial in the slots plus the computed dimensions' children, then the
  *      result, stored.
  * Runs 2, 4 ..., 3 over disjoint subsets give the same output as one plan over every
- * dimension and evaluate the win-pattern groups once, in the finishing run. */
+ * dimension and evaluate the win-pattern groups once, in the finishing run.  Runs 0, 1, 2
+ * and 4 may be repeated with the same output; a finishing run replaces the partial it reads
+ * by the result, so it runs once per partial. */
 pe_status pe_end_slabs_plan_create_dims(int noutputs, const pe_slab* outputs, int nchildren, const pe_slab* children,
                                         uint32_t dims, int accumulate, pe_end_plan** plan);
 

@@ -231,6 +231,7 @@ struct pe_end_plan {
   FusedPlan fused35;             // fused plan of 35-stone output sections (no child data)
   std::vector<EndPlan> passes;   // then line plans, in order
   int64_t nodes = 0;
+  bool finishing = false;        // reads a partial and replaces it by the result: not repeatable
 };
 
 namespace {
@@ -384,6 +385,7 @@ pe_status pe_end_slabs_plan_create_dims(int noutputs, const pe_slab* outputs, in
     return fail(s, err);
   }
   plan->nodes = plan->fused.output_nodes;
+  plan->finishing = accumulate == 3;
   apply_default_form(plan);
   *out = plan;
   return PE_OK;
@@ -431,6 +433,7 @@ pe_status pe_end_set_default_launch(int queue_wgs_per_cu) {
 pe_status pe_end_plan_autotune(pe_end_plan* plan, void* stream, pe_end_tuning* out) {
   PE_GUARD_BEGIN
   if (!plan) return fail(PE_ERR_INVALID, "null plan");
+  if (plan->finishing) return fail(PE_ERR_INVALID, "a finishing pass consumes its partial: it cannot be run repeatedly to time it");
   const hipStream_t s = static_cast<hipStream_t>(stream);
   const int32_t forms[] = {0, 2, 3, 4, 5, 6};
   constexpr int kForms = 6;

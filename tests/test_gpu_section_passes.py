@@ -58,3 +58,28 @@ def test_section_by_passes_matches_fused_plan_and_oracle(counts, groups):
     for blk, e in expect.items():
         idx = bench.slab_index(nb, blk)
         assert np.array_equal(a[idx, :e.size], e), ("passes vs oracle", blk)
+
+
+def test_finishing_pass_is_not_autotuned():
+    """A finishing pass (accumulate 3) replaces the partial it reads by the result, so the
+    launch autotuner, which runs a plan repeatedly, refuses it; the other modes are accepted."""
+    P.init()
+    counts = ((1, 7), (1, 7), (7, 1), (7, 1))
+    nb = [(s + 7) // 8 for s in P.section_shape(counts)]
+    child = P.line_describe(P.make_line(counts, 0, (0, 0, 0))).standard_child.counts
+    cnb = [(s + 7) // 8 for s in P.section_shape(child)]
+    out = hip.Buffer(64 * 4096 * int(np.prod(nb)))
+    kid = hip.Buffer(64 * 4096 * int(np.prod(cnb)))
+    P.fill_seeded(kid.ptr, 4096 * int(np.prod(cnb)), SEED)
+    children = [P.Slab.of(child, [0] * 4, cnb, kid.ptr)]
+    plan = P.Plan.slab_dims(P.Slab.of(counts, [0] * 4, nb, out.ptr), children, 1, accumulate=3)
+    with pytest.raises(P.PentagoError):
+        plan.autotune()
+    plan.close()
+    plan = P.Plan.slab_dims(P.Slab.of(counts, [0] * 4, nb, out.ptr), children, 1, accumulate=2)
+    try:
+        plan.autotune()
+        hip.synchronize()
+    finally:
+        plan.close()
+        P.set_default_launch(0)   # the autotuner's choice would carry over to later plans
