@@ -19,7 +19,8 @@ per-board launches) and serving (p50/p99 latency at Poisson loads) measurements 
 --no-section30 / --no-slices / --no-mid / --no-serve.
 
 --copy-probe BYTES runs one plain device copy of BYTES bytes instead (a kernel of known
-traffic, used to scale the FETCH_SIZE / WRITE_SIZE counters).
+traffic, used to scale the FETCH_SIZE / WRITE_SIZE counters); --gather-probe runs the fused
+kernel's access pattern alone (hashed 64-byte gathers, no arithmetic) for its ceiling.
 """
 import argparse
 import json
@@ -249,6 +250,50 @@ def copy_probe(nbytes):
     torch.cuda.synchronize()
     assert bool((y[-4:] == 1).all())
     print(json.dumps({"copy_probe_bytes": 4 * n, "read": 4 * n, "written": 4 * n}), flush=True)
+
+
+def gather_probe(nodes=976289792, src_nodes=1 << 31, steps=3, windows=(0, 40 * 4096)):
+    """The fused kernel's memory access pattern without its arithmetic
+    (pentago_amd/libpentago_probe.so, a measurement probe outside the engine): one thread per
+    output node of the headline launch reads 12 hashed 64-byte nodes of a 137 GB buffer
+    (lanes paired on 128-byte lines, three reads at a time) and writes one node.  Window 0
+    hashes every read over the whole buffer; a window of 40 blocks of nodes (10.5 MB, a child
+    line of the headline's sections) confines each block's three reads per group to one
+    hashed stretch of that size, as a block's moves along one dimension read one child line.
+    Its rate is the ceiling the fused kernel's own gathers meet; HIP events on the launch
+    stream, best of `steps` runs after one warm-up per window."""
+    import ctypes as C
+    from pentago_amd import hip
+    L = C.CDLL(os.path.join(ROOT, "pentago_amd", "libpentago_probe.so"))
+    L.pe_probe_gather.restype = C.c_int
+    L.pe_probe_gather.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_int64, C.c_uint64, C.c_void_p]
+    src = hip.Buffer(64 * src_nodes)
+    src.zero()
+    dst = hip.Buffer(64 * nodes)
+    stream, e0, e1 = hip.Stream(), hip.Event(), hip.Event()
+    hip.synchronize()
+    res = {"nodes": nodes, "reads_per_node": 12, "src_GB": 64 * src_nodes / 1e9, "windows": {}}
+    requested = 64 * 13 * nodes   # 12 reads and one write of 64 bytes per node
+    for window in windows:
+        def run(seed):
+            hip.check(L.pe_probe_gather(src.ptr, src_nodes, dst.ptr, nodes, window, seed, stream.ptr),
+                      "pe_probe_gather")
+
+        run(1)
+        stream.synchronize()
+        ms = []
+        for k in range(steps):
+            e0.record(stream)
+            run(2 + k)
+            e1.record(stream)
+            e1.synchronize()
+            ms.append(e0.elapsed_ms(e1))
+        best = min(ms)
+        res["windows"][str(window)] = {"window_nodes": window or src_nodes, "ms": ms, "best_ms": best,
+                                       "requested_GBps": requested / (best / 1e3) / 1e9}
+        log("gather probe, window %d nodes: %s" % (window or src_nodes, res["windows"][str(window)]))
+    print(json.dumps({"gather_probe": res}), flush=True)
+    return res
 
 
 SECTION30 = ((3, 4), (4, 4), (4, 4), (4, 3))   # standard form of (4,4)(4,3)(3,4)(4,4)
@@ -875,6 +920,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", type=int, default=0, help="0: fused block kernel, 1: four line passes")
     ap.add_argument("--copy-probe", type=int, default=0, help="run one plain copy of this many bytes instead")
+    ap.add_argument("--gather-probe", action="store_true",
+                    help="run the gather probe (the fused kernel's access pattern alone) instead")
     ap.add_argument("--no-mid", action="store_true", help="skip the batched midsolve measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-end", action="store_true", help="skip the end-engine measurement (profiling the others)")
@@ -902,6 +949,9 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.gather_probe:
+        gather_probe()
         return
 
     import pentago_amd as P
