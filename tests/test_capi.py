@@ -111,3 +111,17 @@ def test_slab_dims_arguments_checked_on_the_host():
     full = P.Slab.of(((4, 4), (4, 4), (4, 4), (4, 3)), [0] * 4, [1] * 4, 1 << 20)
     with pytest.raises(P.PentagoError):
         P.Plan.slab_dims(full, [], 1, accumulate=3)
+
+
+def test_gather_probe_library_checks_its_arguments():
+    # the measurement probe is its own library (the engine does not export it); bad shapes
+    # are refused before any launch: odd or tiny buffers, windows outside the buffer, no nodes
+    lib = C.CDLL(os.path.join(os.path.dirname(P.LIB_PATH), "libpentago_probe.so"))
+    assert not hasattr(C.CDLL(P.LIB_PATH), "pe_probe_gather")
+    f = lib.pe_probe_gather
+    f.restype = C.c_int
+    f.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_int64, C.c_uint64, C.c_void_p]
+    fake = 1 << 20   # never dereferenced: every call below is refused first
+    for src_nodes, n, window in ((3, 1, 0), (0, 1, 0), (8, 0, 0), (8, 1, 10), (8, 1, 3), (8, 1 << 32, 0)):
+        assert f(fake, src_nodes, fake, n, window, 1, None) != 0, (src_nodes, n, window)
+    assert f(None, 8, fake, 1, 0, 1, None) != 0
