@@ -1,9 +1,9 @@
 """Whole sections computed in passes over subsets of their open dimensions (the form the
 benchmark's slice-30 section runs in: output resident, only the pass's child sections
-resident, fused launches with the other dimensions closed, later passes OR-accumulating into
-the slab slots) against the fused slab plan with every child resident, on every block, and
-against the oracle on sampled blocks (pentago/end/compute.cc:219-371 per line,
-pentago/end/block_store.cc:173-229 merge)."""
+resident, fused launches with the other dimensions closed; the first pass stores a partial in
+the slab slots, middle passes add to it and the last finishes it) against the fused slab plan
+with every child resident, on every block, and against the oracle on sampled blocks
+(pentago/end/compute.cc:219-371 per line, pentago/end/block_store.cc:173-229 merge)."""
 import numpy as np
 import pytest
 
