@@ -83,3 +83,41 @@ def test_finishing_pass_is_not_autotuned():
     finally:
         plan.close()
         P.set_default_launch(0)   # the autotuner's choice would carry over to later plans
+
+
+@pytest.mark.parametrize("groups", [[[0], [1], [2], [3]], [[1, 2, 3], [0]]])
+def test_or_merged_passes_match_fused_plan(groups):
+    """accumulate 0 then 1: each pass a complete result over its dimensions, OR-merged into the
+    slots (the block store's merge, pentago/end/block_store.cc:173-229), equals the fused plan
+    over every dimension on every block."""
+    P.init()
+    counts = ((1, 7), (1, 7), (7, 1), (7, 1))
+    shape = P.section_shape(counts)
+    nb = [(s + 7) // 8 for s in shape]
+    kids = {}
+    for d in range(4):
+        child = P.line_describe(P.make_line(counts, d, (0, 0, 0))).standard_child.counts
+        if child not in kids:
+            cnb = [(s + 7) // 8 for s in P.section_shape(child)]
+            buf = hip.Buffer(64 * 4096 * int(np.prod(cnb)))
+            P.fill_seeded(buf.ptr, 4096 * int(np.prod(cnb)), SEED + sum(c * 10 ** i for i, c in enumerate(sum(child, ()))))
+            kids[child] = (cnb, buf)
+    slabs = [P.Slab.of(c, [0] * 4, cnb, buf.ptr) for c, (cnb, buf) in kids.items()]
+    nbytes = 64 * 4096 * int(np.prod(nb))
+    merged, fused = hip.Buffer(nbytes), hip.Buffer(nbytes)
+    for k, group in enumerate(groups):
+        print("pass", group, "accumulate", int(k > 0), flush=True)
+        plan = P.Plan.slab_dims(P.Slab.of(counts, [0] * 4, nb, merged.ptr), slabs, sum(1 << d for d in group),
+                                accumulate=int(k > 0))
+        plan.run()
+        hip.synchronize()
+        plan.close()
+    plan = P.Plan.slab(P.Slab.of(counts, [0] * 4, nb, fused.ptr), slabs, 0)
+    plan.run()
+    hip.synchronize()
+    plan.close()
+    a = merged.download(np.zeros(nbytes // 8, dtype=np.uint64)).reshape(-1, 4096 * 8)
+    b = fused.download(np.zeros(nbytes // 8, dtype=np.uint64)).reshape(-1, 4096 * 8)
+    for idx, blk in enumerate(np.ndindex(*nb)):
+        m = 8 * int(np.prod([min(8, shape[x] - 8 * blk[x]) for x in range(4)]))
+        assert np.array_equal(a[idx, :m], b[idx, :m]), blk
