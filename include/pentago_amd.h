@@ -373,10 +373,10 @@ pe_status pe_mid_plan_destroy(pe_mid_plan* plan);
  * configuration would use, or PE_ERR_CAPACITY.  No device calls. */
 pe_status pe_mid_schedule_info(int n, const pe_high_board* boards, const pe_mid_config* config, int64_t* launches,
                                int64_t* workspace_bytes);
-/* Time candidate configurations on these boards, one warm run of the whole batch each: the split
- * small_spots in {12, 14, 16} at 512 threads, then 256 and 1024 threads at the best split,
- * then the winner replayed as a captured hipGraph against eager launches; the fastest becomes
- * the current device's default.  Hand in the batch itself (or one like it): the best split
+/* Time candidate configurations on these boards (a warm-up, then the faster of two runs of the
+ * whole batch each): the split small_spots in {12, 14, 16} at 512 threads, then 256 and 1024
+ * threads at the best split, then the winner replayed as a captured hipGraph against eager
+ * launches; the fastest becomes the current device's default.  Hand in the batch itself (or one like it): the best split
  * depends on how many boards of each size there are.  The chosen configuration is returned
  * through *best (use_graph 1 or -1). */
 pe_status pe_mid_autotune(int n, const pe_high_board* boards, int64_t workspace_budget, pe_mid_config* best);

@@ -661,7 +661,7 @@ def mid_schedule_info(boards, workspace_budget, small_spots=14, workgroups=1024,
 
 def mid_autotune(boards, workspace_budget=0):
     """pe_mid_autotune: time candidate configurations (small/large split, threads per
-    small-board workgroup, graph replay) with one run of these boards each; the best becomes
+    small-board workgroup, graph replay), the faster of two runs of these boards each; the best becomes
     the device default.  Returns the chosen configuration."""
     n = len(boards)
     arr = (HighBoard * n)(*boards)
