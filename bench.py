@@ -418,6 +418,23 @@ def slab_index(nb, b):
     return idx
 
 
+def pass_groups(dims, child, cbytes, room):
+    """The open dimensions grouped greedily, in order, into as few passes as fit in `room`
+    bytes (a pass holds its dimensions' child sections, a child shared by two dimensions
+    once), ordered from the most dimensions to the fewest: the finishing pass carries the
+    win patterns and is cheapest with the fewest dimensions' gathers beside them (slice 30:
+    [1, 2, 3] then [0], 264 ms; the other order 282 ms).  child: {dimension: child section},
+    cbytes: {child section: bytes}."""
+    groups = []
+    for d in dims:
+        if groups and sum(cbytes[c] for c in {child[x] for x in groups[-1] + [d]}) <= room:
+            groups[-1].append(d)
+        else:
+            groups.append([d])
+    groups.sort(key=len, reverse=True)
+    return groups
+
+
 def section_by_passes(P, hip, counts, samples, stream, seed_base=0x5EC30000, groups=None, headroom=0.92):
     """A whole section computed in passes over subsets of its open dimensions: the output
     (every block, one 4096-node slot each) stays resident, and each pass holds only its
@@ -449,17 +466,7 @@ def section_by_passes(P, hip, counts, samples, stream, seed_base=0x5EC30000, gro
     cnb = {c: [(s + 7) // 8 for s in P.section_shape(c)] for c in child.values()}
     cbytes = {c: 64 * 4096 * int(np.prod(n)) for c, n in cnb.items()}
     if groups is None:
-        room = headroom * hip.mem_info()[0] - out_bytes
-        groups = []
-        for d in dims:
-            if groups and sum(cbytes[c] for c in {child[x] for x in groups[-1] + [d]}) <= room:
-                groups[-1].append(d)
-            else:
-                groups.append([d])
-        # the finishing pass carries the win patterns; it is cheapest with the fewest
-        # dimensions' gathers beside them (slice 30: [1, 2, 3] then [0], 264 ms; the other
-        # order 282 ms)
-        groups.sort(key=len, reverse=True)
+        groups = pass_groups(dims, child, cbytes, headroom * hip.mem_info()[0] - out_bytes)
     out = hip.Buffer(out_bytes)
     out_slab = P.Slab.of(counts, [0] * 4, nb, out.ptr)
     ev0, ev1 = hip.Event(), hip.Event()
