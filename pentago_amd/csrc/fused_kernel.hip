@@ -133,12 +133,13 @@ __device__ __forceinline__ void fused_block(const BlockDev& B, const SecDev& S, 
   const int bline[4] = {B.line[0], B.line[1], B.line[2], B.line[3]};
   const int bs0 = min(8, S.rmin_size[0] - 8 * bb[0]), bs1 = min(8, S.rmin_size[1] - 8 * bb[1]);
   const int bs2 = min(8, S.rmin_size[2] - 8 * bb[2]), bs3 = min(8, S.rmin_size[3] - 8 * bb[3]);
-  // the block's field entries of quadrants 2 and 3 (eight each) staged in LDS
   // B.accumulate: 0 store the result, 1 OR it into the output; partial passes (sections run
   // in passes over subsets of their dimensions): 2 store the children's OR only, 4 add to
   // the partial in the output slots, 3 finish from it (the result, stored)
   const int mode = B.accumulate;
   const bool partial_out = !S35 && (mode == 2 || mode == 4);
+  // the block's field entries of quadrants 2 and 3 (eight each) staged in LDS (the win
+  // patterns need them; a partial pass does not)
   __shared__ uint64_t sf[2][8][60];
   if (!partial_out) {   // uniform over the workgroup (one block)
     for (int i = threadIdx.x; i < 2 * 8 * 60; i += kThreads) {
@@ -584,27 +585,42 @@ static void build_slab_body(const pe_slab& out, int nch, const pe_slab* ch, uint
     sec.field_off[a] = field_pool_off(a, sec.turn, accumulate == 3 || ((dims >> a) & 1), s.c[a][0], s.c[a][1]);
     if (!(dims >> a & 1)) { sec.move_off[a] = -1; sec.nmoves[a] = 0; }
   }
+  // blocks in C order over the box, except that the last computed dimension runs innermost
+  // (with every dimension open, the plain C order): consecutive blocks of an XCD's run then
+  // lie along that dimension's lines, whose child reads the L2 can share
+  int inner = 3;
+  for (int d = 0; d < 4; d++)
+    if (s.c[d][0] + s.c[d][1] < 9 && (dims >> d & 1)) inner = d;
+  int ord[4], nord = 0;
+  for (int a = 0; a < 4; a++)
+    if (a != inner) ord[nord++] = a;
+  ord[3] = inner;
+  int64_t total = 1;
+  for (int a = 0; a < 4; a++) total *= std::max(0, out.hi[a] - out.lo[a]);
   int b[4];
-  for (b[0] = out.lo[0]; b[0] < out.hi[0]; b[0]++)
-    for (b[1] = out.lo[1]; b[1] < out.hi[1]; b[1]++)
-      for (b[2] = out.lo[2]; b[2] < out.hi[2]; b[2]++)
-        for (b[3] = out.lo[3]; b[3] < out.hi[3]; b[3]++) {
-          BlockDev B;
-          B.sec = 0;
-          B.accumulate = accumulate;
-          int n = 1;
-          for (int a = 0; a < 4; a++) { B.b[a] = b[a]; n *= std::min(8, sec.rmin_size[a] - 8 * b[a]); }
-          for (int d = 0; d < 4; d++) {
-            B.line[d] = -1;
-            if (s.c[d][0] + s.c[d][1] >= 9 || !(dims >> d & 1)) continue;
-            int o[3], kk = 0;
-            for (int a = 0; a < 4; a++) if (a != d) o[kk++] = b[a];
-            B.line[d] = index.at(std::make_tuple(d, o[0], o[1], o[2]));
-          }
-          B.out = static_cast<uint64_t*>(out.data) + 8 * 4096 * slab_index(out, b);
-          r->blocks.push_back(B);
-          r->nodes += n;
-        }
+  for (int64_t t = 0; t < total; t++) {
+    int64_t rest = t;
+    for (int j = 3; j >= 0; j--) {
+      const int a = ord[j], e = out.hi[a] - out.lo[a];
+      b[a] = out.lo[a] + int(rest % e);
+      rest /= e;
+    }
+    BlockDev B;
+    B.sec = 0;
+    B.accumulate = accumulate;
+    int n = 1;
+    for (int a = 0; a < 4; a++) { B.b[a] = b[a]; n *= std::min(8, sec.rmin_size[a] - 8 * b[a]); }
+    for (int d = 0; d < 4; d++) {
+      B.line[d] = -1;
+      if (s.c[d][0] + s.c[d][1] >= 9 || !(dims >> d & 1)) continue;
+      int o[3], kk = 0;
+      for (int a = 0; a < 4; a++) if (a != d) o[kk++] = b[a];
+      B.line[d] = index.at(std::make_tuple(d, o[0], o[1], o[2]));
+    }
+    B.out = static_cast<uint64_t*>(out.data) + 8 * 4096 * slab_index(out, b);
+    r->blocks.push_back(B);
+    r->nodes += n;
+  }
 }
 
 // Worker-thread entry: no exception leaves the thread (a host allocation failure or a bad
