@@ -21,6 +21,9 @@ SEED = 0x7E570000
     (((1, 7), (1, 7), (7, 1), (7, 1)), [[0, 3], [1, 2]]),
     (((3, 5), (2, 6), (5, 3), (6, 2)), [[0], [1, 2, 3]]),
     (((3, 5), (2, 6), (5, 3), (6, 2)), None),
+    # quadrant 0 full (dimension 0 closed): the finishing pass's fields for it carry no moves
+    (((5, 4), (1, 7), (7, 1), (2, 3)), [[1], [2], [3]]),
+    (((5, 4), (1, 7), (7, 1), (2, 3)), [[3], [1, 2]]),
 ])
 def test_section_by_passes_matches_fused_plan_and_oracle(counts, groups):
     P.init()
